@@ -1,0 +1,239 @@
+/*
+ * ORACLE -- test infrastructure only.  Nothing in the product (vir_amd/) may
+ * link, load or call this file; only tests/, the smoke check and the CPU
+ * baseline leg of bench.py use it, and only as the checker.
+ *
+ * Plain-C restatement of the reference's Siddon system-matrix builder
+ * `siddons()` in sys_mat/siddon.py:636-833 (grid planes from
+ * core/core_funcs.py:62-81 and :235-265).  It follows the reference's
+ * algorithm as written there -- per ray: compute ALL plane-crossing alphas,
+ * sort them, trim the ends, then emit one (voxel, length) entry per
+ * consecutive pair -- and reproduces numpy's dtype behaviour bit for bit:
+ *
+ *   - every array op on the f32 ray data is an IEEE single op, unfused
+ *     (build with -ffp-contract=off);
+ *   - the start-plane index i0 is evaluated in double, because numpy
+ *     promotes int64 (nPixels+1) minus a float32 array to float64
+ *     (siddon.py:740-741);
+ *   - the merged alpha list is float64 for every ray that has a zero
+ *     direction component, because np.array([]) is float64 and
+ *     np.concatenate promotes (siddon.py:776-787);
+ *   - np.isclose is |x-y| <= atol + rtol*|y| (& isfinite(y)) | x==y,
+ *     evaluated in the dtype of its operands (numpy/_core/numeric.py);
+ *   - np.linalg.norm(axis=-1) of an (...,3) float32 array is
+ *     sqrtf((x*x + y*y) + z*z) (checked against numpy in tests);
+ *   - astype(int32) truncates toward zero; values outside int32 become
+ *     INT32_MIN as on x86-64.
+ *
+ * This file is deliberately a different algorithm from the GPU kernels
+ * (which merge three monotone plane streams on the fly): sort-then-scan
+ * versus streaming merge, so agreement between the two is evidence.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define RTOL_F 1e-5f
+#define RTOL_D 1e-5
+
+static int cmp_double(const void *a, const void *b) {
+    double x = *(const double *)a, y = *(const double *)b;
+    /* numpy sorts NaN last */
+    int xn = isnan(x), yn = isnan(y);
+    if (xn || yn) return xn - yn;
+    return (x > y) - (x < y);
+}
+
+static int32_t to_i32(double v) {
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return INT32_MIN;
+    return (int32_t)v;
+}
+
+/* np.isclose on float32 operands with python-float atol/rtol */
+static int isclose_f(float x, float y, float atol) {
+    float tol = atol + RTOL_F * fabsf(y);
+    return ((fabsf(x - y) <= tol) && isfinite(y)) || (x == y);
+}
+
+/* np.isclose on float64 operands */
+static int isclose_d(double x, double y, double atol) {
+    double tol = atol + RTOL_D * fabs(y);
+    return ((fabs(x - y) <= tol) && isfinite(y)) || (x == y);
+}
+
+/* Python slice normalisation for seq[start:stop] on a sequence of length L */
+static void py_slice(int64_t start, int64_t stop, int64_t L, int64_t *s, int64_t *e) {
+    if (start < 0) { start += L; if (start < 0) start = 0; }
+    if (start > L) start = L;
+    if (stop < 0) { stop += L; if (stop < 0) stop = 0; }
+    if (stop > L) stop = L;
+    *s = start;
+    *e = stop < start ? start : stop;
+}
+
+/* Grid plane coordinates: boundspace(n, d, c, float32) core_funcs.py:62-81
+ * = float32(linspace(-n, n, n+1)) * d / 2.0 + c, each op in float32. */
+void oracle_planes(int64_t n, float d, float o, float *Xb) {
+    for (int64_t k = 0; k <= n; ++k) {
+        float lin = (float)(2 * k - n);
+        float t = lin * d;
+        t = t / 2.0f;
+        Xb[k] = t + o;
+    }
+}
+
+/*
+ * Trace nrays rays.  Outputs per ray r the entries [off[r], off[r+1]) of
+ * (ix, iy, iz, len).  flags[r] bit 0 = the ray produced an index outside
+ * the grid (the reference raises ValueError in ravel/flat modes); bit 1 =
+ * the ray passed the entry/exit gate (the reference then stores a list,
+ * possibly empty, instead of None).
+ * Returns total entries, or -1 if cap would be exceeded.
+ */
+int64_t oracle_siddons(int64_t nrays, const float *src, const float *trg,
+                       const int64_t n[3], const float d[3], const float o[3],
+                       double epsilon,
+                       int32_t *ox, int32_t *oy, int32_t *oz, float *olen,
+                       int64_t *off, uint8_t *bad, int64_t cap) {
+    float *planes[3];
+    for (int a = 0; a < 3; ++a) {
+        planes[a] = (float *)malloc(sizeof(float) * (size_t)(n[a] + 1));
+        oracle_planes(n[a], d[a], o[a], planes[a]);
+    }
+    const float eps_f = (float)epsilon;
+    float p0[3], pN[3];
+    for (int a = 0; a < 3; ++a) { p0[a] = planes[a][0]; pN[a] = planes[a][n[a]]; }
+
+    int64_t maxlen = 2 + (n[0] + 1) + (n[1] + 1) + (n[2] + 1);
+    double *alist = (double *)malloc(sizeof(double) * (size_t)maxlen);
+    float *flist = (float *)malloc(sizeof(float) * (size_t)maxlen);
+    int64_t total = 0;
+    off[0] = 0;
+
+    for (int64_t r = 0; r < nrays; ++r) {
+        const float *s = src + 3 * r, *t = trg + 3 * r;
+        float dST[3];
+        for (int a = 0; a < 3; ++a) dST[a] = t[a] - s[a];
+        float sq0 = dST[0] * dST[0], sq1 = dST[1] * dST[1], sq2 = dST[2] * dST[2];
+        float distance = sqrtf((sq0 + sq1) + sq2);
+        bad[r] = 0;
+
+        /* siddon.py:717-720 */
+        float alpha0[3], alphaN[3];
+        for (int a = 0; a < 3; ++a) {
+            float num0 = p0[a] - s[a], numN = pN[a] - s[a];
+            if (dST[a] != 0.0f) { alpha0[a] = num0 / dST[a]; alphaN[a] = numN / dST[a]; }
+            else { alpha0[a] = num0 / eps_f; alphaN[a] = numN / eps_f; }
+        }
+        /* siddon.py:725-736 */
+        float amin = 0.0f, amax = 1.0f;
+        int nan_min = 0, nan_max = 0;
+        for (int a = 0; a < 3; ++a) {
+            float mn = 0.0f, mx = 1.0f;
+            if (alpha0[a] < alphaN[a]) { mn = alpha0[a]; mx = alphaN[a]; }
+            else if (alpha0[a] > alphaN[a]) { mn = alphaN[a]; mx = alpha0[a]; }
+            if (isnan(mn)) nan_min = 1;
+            if (isnan(mx)) nan_max = 1;
+            if (mn > amin) amin = mn;
+            if (mx < amax) amax = mx;
+        }
+        if (nan_min) amin = NAN;
+        if (nan_max) amax = NAN;
+
+        /* siddon.py:740-751 */
+        int32_t i0[3], iN[3];
+        for (int a = 0; a < 3; ++a) {
+            float q0p = ((pN[a] - amin * dST[a]) - s[a]) / d[a];
+            float q0n = ((pN[a] - amax * dST[a]) - s[a]) / d[a];
+            double v0p = (double)(n[a] + 1) - (double)q0p;
+            double v0n = (double)(n[a] + 1) - (double)q0n;
+            float vNp = ((s[a] + amax * dST[a]) - p0[a]) / d[a];
+            float vNn = ((s[a] + amin * dST[a]) - p0[a]) / d[a];
+            double r0p = rint(v0p), r0n = rint(v0n);
+            float rNp = rintf(vNp), rNn = rintf(vNn);
+            int32_t i0p = to_i32(isclose_d(v0p, r0p, epsilon) ? r0p : floor(v0p));
+            int32_t i0n = to_i32(isclose_d(v0n, r0n, epsilon) ? r0n : floor(v0n));
+            int32_t iNp = to_i32(isclose_f(vNp, rNp, eps_f) ? (double)rNp : (double)ceilf(vNp));
+            int32_t iNn = to_i32(isclose_f(vNn, rNn, eps_f) ? (double)rNn : (double)ceilf(vNn));
+            i0[a] = dST[a] > 0.0f ? i0p : i0n;
+            iN[a] = dST[a] > 0.0f ? iNp : iNn;
+        }
+
+        /* gate, siddon.py:759-760 */
+        int hit = (amax > amin) && !isclose_f(amax, amin, eps_f);
+        if (hit) {
+            bad[r] |= 2;
+            int f64mode = (dST[0] == 0.0f) || (dST[1] == 0.0f) || (dST[2] == 0.0f);
+            int64_t L = 0;
+            if (f64mode) { alist[L++] = amin; alist[L++] = amax; }
+            else { flist[L++] = amin; flist[L++] = amax; }
+            for (int a = 0; a < 3; ++a) {
+                if (dST[a] == 0.0f) continue;
+                int64_t ks, ke;
+                py_slice(i0[a], iN[a], n[a] + 1, &ks, &ke);
+                for (int64_t k = ks; k < ke; ++k) {
+                    float al = (planes[a][k] - s[a]) / dST[a];
+                    if (f64mode) alist[L++] = al; else flist[L++] = al;
+                }
+            }
+            /* sort ascending; values are identical bits whether held as
+             * float or double, so the order of the multiset is unique up
+             * to equal values */
+            if (!f64mode) for (int64_t i = 0; i < L; ++i) alist[i] = flist[i];
+            qsort(alist, (size_t)L, sizeof(double), cmp_double);
+            int64_t b = 0, e = L;
+            /* trim, siddon.py:789-793 (default atol=1e-8) */
+            if (f64mode) {
+                if (isclose_d(alist[0], alist[1], 1e-8)) b = 1;
+                if (isclose_d(alist[e - 1], alist[e - 2], 1e-8)) e -= 1;
+            } else {
+                if (isclose_f((float)alist[0], (float)alist[1], 1e-8f)) b = 1;
+                if (isclose_f((float)alist[e - 1], (float)alist[e - 2], 1e-8f)) e -= 1;
+            }
+            for (int64_t i = b; i + 1 < e; ++i) {
+                int32_t idx[3];
+                float len;
+                if (f64mode) {
+                    double A0 = alist[i], A1 = alist[i + 1];
+                    double dA = A1 - A0;
+                    double m = 0.5 * (A1 + A0);
+                    for (int a = 0; a < 3; ++a) {
+                        double pos = ((double)s[a] + m * (double)dST[a]) - (double)p0[a];
+                        idx[a] = to_i32(pos / (double)d[a]);
+                    }
+                    len = (float)((double)distance * dA);
+                } else {
+                    float A0 = (float)alist[i], A1 = (float)alist[i + 1];
+                    float dA = A1 - A0;
+                    float m = 0.5f * (A1 + A0);
+                    for (int a = 0; a < 3; ++a) {
+                        float pos = (s[a] + m * dST[a]) - p0[a];
+                        idx[a] = to_i32((double)(pos / d[a]));
+                    }
+                    len = distance * dA;
+                }
+                if (!(len > 0.0f)) continue;
+                if (total >= cap) {
+                    for (int a = 0; a < 3; ++a) free(planes[a]);
+                    free(alist); free(flist);
+                    return -1;
+                }
+                for (int a = 0; a < 3; ++a)
+                    if (idx[a] < 0 || idx[a] >= n[a]) bad[r] |= 1;
+                ox[total] = idx[0]; oy[total] = idx[1]; oz[total] = idx[2];
+                olen[total] = len;
+                ++total;
+            }
+        }
+        off[r + 1] = total;
+    }
+    for (int a = 0; a < 3; ++a) free(planes[a]);
+    free(alist); free(flist);
+    return total;
+}
+
+/* Upper bound on entries per ray (planes + 2 bounds - 1 segments). */
+int64_t oracle_siddons_maxlen(const int64_t n[3]) {
+    return (n[0] + 1) + (n[1] + 1) + (n[2] + 1) + 1;
+}
