@@ -1,0 +1,136 @@
+/*
+ * vir_amd -- C ABI of the MI355X (gfx950) projector / SIRT library.
+ *
+ * Conventions shared by every entry point:
+ *   - every array argument is a DEVICE pointer (hipMalloc'd or a torch
+ *     tensor's data_ptr()), C-contiguous, float32 / int32 / int64 as typed;
+ *   - work is enqueued on `stream` (a hipStream_t, NULL = default stream) and
+ *     the call returns without synchronising, unless documented otherwise;
+ *   - return 0 on success, VIR_EINVAL for an invalid argument (nothing was
+ *     launched), VIR_EHIP for a HIP runtime error; vir_last_error() returns
+ *     the calling thread's last message;
+ *   - outputs are overwritten unless VIR_ACCUMULATE is passed, in which case
+ *     results are added to what the output already holds;
+ *   - volumes are (nX, nY, nZ) C-order, voxel (x, y, z) at (x*nY + y)*nZ + z,
+ *     the ravel order of the reference's np.ravel_multi_index.
+ *
+ * Reference interfaces replaced are cited per function as file:line in the
+ * vargasp/vir tree.
+ */
+#ifndef VIR_AMD_H
+#define VIR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIR_OK 0
+#define VIR_EINVAL (-1)
+#define VIR_EHIP (-2)
+
+#define VIR_ACCUMULATE 1u   /* add into the output instead of overwriting  */
+#define VIR_SEQUENTIAL 2u   /* plain sequential f32 sums in reference order */
+
+/* Voxel grid: Grid3d(nPixels, dPixels, origin, float32) of
+ * core/core_funcs.py:235-265 plus the snapping epsilon of
+ * siddons(..., epsilon) (sys_mat/siddon.py:636). */
+typedef struct vir_grid {
+    int32_t n[3];
+    float d[3];
+    float o[3];
+    double eps;
+} vir_grid;
+
+int vir_version(void);
+const char *vir_last_error(void);
+
+/* ---------------------------------------------------------------- Siddon */
+
+/* Pass 1 of siddons(src, trg, ..., flat=True) (sys_mat/siddon.py:636-833):
+ * per ray, the number of (voxel, length) entries counts[r] and flags[r]
+ * (bit 0: an index fell outside the grid -- the reference raises
+ * ValueError in ravel/flat modes, siddon.py:815,821; bit 1: the ray passed
+ * the entry/exit gate, siddon.py:759-760).  src, trg: (nrays, 3) float32. */
+int vir_siddon_count(const float *src, const float *trg, int64_t nrays,
+                     const vir_grid *grid, int32_t *counts, uint8_t *flags,
+                     void *stream);
+
+/* Pass 2: write the entries of ray r at [offsets[r], offsets[r+1]).
+ * Any of ravel_idx / ix / iy / iz may be NULL.  ravel_idx gets the
+ * C-order flat voxel index (int32, as flat_ind of siddon.py:815) of
+ * in-grid entries and -1 for out-of-grid ones; ix/iy/iz get the
+ * unravelled int32 indices (ravel=False output, siddon.py:823-824). */
+int vir_siddon_fill(const float *src, const float *trg, int64_t nrays,
+                    const vir_grid *grid, const int64_t *offsets,
+                    int32_t *ravel_idx, int32_t *ix, int32_t *iy, int32_t *iz,
+                    float *len, void *stream);
+
+/* On-the-fly forward projection sino[r] = sum_j vol[idx_j] * len_j over the
+ * ray's Siddon entries, with no stored matrix.  Same result as
+ * siddons(flat=True) followed by the CSR gather of
+ * projection/proj_numba.py:11-20 / proj_cuda.py:14-27; compensated f32 sum.
+ * err_count (may be NULL) receives the number of out-of-grid entries. */
+int vir_siddon_fp(const float *src, const float *trg, int64_t nrays,
+                  const vir_grid *grid, const float *vol, float *sino,
+                  uint32_t flags, int32_t *err_count, void *stream);
+
+/* Exact adjoint of vir_siddon_fp for rays laid out as
+ * (nviews, nu, nv) -- ray (a, iu, iv) at index (a*nu + iu)*nv + iv --
+ * whose view geometry is given by per-view 3x4 projection matrices
+ * proj[a*12 .. a*12+11] mapping a world point X to the fractional ray
+ * coordinates (iu, iv) = (P0.X~ / P2.X~, P1.X~ / P2.X~) with P2.X~ > 0 in
+ * front of the source.  Voxel-brick gather: every brick pulls the rays of its
+ * detector footprint, re-traces them with the forward projector's arithmetic
+ * and accumulates into on-chip memory in conflict-free colour phases; no
+ * atomics, deterministic.  Replaces the scatter backprojectors
+ * projection/proj.py:119-150 and projection_c.c:176-219.
+ * workspace: device scratch of vir_siddon_bp_workspace() bytes (may be NULL
+ * when that size is 0). */
+int64_t vir_siddon_bp_workspace(const vir_grid *grid, int32_t nviews);
+int vir_siddon_bp(const float *src, const float *trg, int32_t nviews,
+                  int32_t nu, int32_t nv, const float *proj,
+                  const vir_grid *grid, const float *sino, float *vol,
+                  uint32_t flags, void *workspace, void *stream);
+
+/* ------------------------------------------------- stored matrix (CSR) */
+
+/* sino[r] = sum_{e in [off[r], off[r+1])} vol[ind[e]] * len[e]
+ * (projection_c.c:101-168, proj_numba.py:11-32, proj_cuda.py:14-27).
+ * VIR_SEQUENTIAL sums in entry order in plain f32 exactly like
+ * forward_proj_ravel_rays_struct; the default is a compensated sum. */
+int vir_csr_fp(const int32_t *ind, const float *len, const int64_t *off,
+               int64_t nrays, const float *vol, float *sino, uint32_t flags,
+               void *stream);
+
+/* Transpose a CSR system matrix into voxel-major (CSC) form: colptr
+ * (nvox+1, int64), row (nnz, int32) and clen (nnz, float32), entries of each
+ * voxel kept in CSR order (a stable counting sort).  Synchronous. */
+int vir_csr_transpose(const int32_t *ind, const float *len, const int64_t *off,
+                      int64_t nrays, int64_t nnz, int64_t nvox,
+                      int64_t *colptr, int32_t *row, float *clen, void *stream);
+
+/* vol[v] = sum_{e in [colptr[v], colptr[v+1])} sino[row[e]] * clen[e]:
+ * the backprojection of projection_c.c:176-219 as an atomic-free gather.
+ * VIR_SEQUENTIAL reproduces back_proj_ravel_rays_struct's f32 rounding. */
+int vir_csc_bp(const int64_t *colptr, const int32_t *row, const float *clen,
+               int64_t nvox, const float *sino, float *vol, uint32_t flags,
+               void *stream);
+
+/* ------------------------------------------------------------- SIRT ops */
+
+/* r[i] = W[i] != 0 ? (b[i] - r[i]) / W[i] : r[i]   (sys_mat/sirt_dd.py:23-26,
+ * tests/Test_SIRT/sirt_test.py:29) */
+int vir_sirt_residual(const float *b, float *r, const float *W, int64_t n,
+                      void *stream);
+
+/* x[i] += gamma * (W[i] != 0 ? u[i] / W[i] : u[i])   (sirt_dd.py:30-33,72;
+ * sirt_test.py:33-34) */
+int vir_sirt_update(float *x, const float *u, const float *W, float gamma,
+                    int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VIR_AMD_H */

@@ -24,14 +24,13 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _lib = None
 
 
-def _load():
-    global _lib
-    if _lib is None:
-        path = os.path.join(_HERE, "liboracle.so")
-        if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
-        lib = ctypes.CDLL(path)
-        P = ctypes.c_void_p
+def load_library(path):
+    """Open an oracle build (the default one, or a planted-mistake variant)."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `make -C oracle`")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    if True:
         lib.oracle_siddons.restype = ctypes.c_int64
         lib.oracle_siddons.argtypes = [ctypes.c_int64, P, P, P, P, P,
                                        ctypes.c_double, P, P, P, P, P, P,
@@ -39,7 +38,13 @@ def _load():
         lib.oracle_planes.restype = None
         lib.oracle_planes.argtypes = [ctypes.c_int64, ctypes.c_float,
                                       ctypes.c_float, P]
-        _lib = lib
+    return lib
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        _lib = load_library(os.path.join(_HERE, "liboracle.so"))
     return _lib
 
 
@@ -69,9 +74,9 @@ def planes(n, d, o):
     return out
 
 
-def trace(src, trg, nPixels=128, dPixels=1.0, origin=0.0, epsilon=1e-5):
-    """Raw trace: returns (shape, x, y, z, len, off int64, bad uint8)."""
-    lib = _load()
+def trace(src, trg, nPixels=128, dPixels=1.0, origin=0.0, epsilon=1e-5, lib=None):
+    """Raw trace: returns (shape, x, y, z, len, off int64, flags uint8, n)."""
+    lib = lib or _load()
     n, d, o = grid_params(nPixels, dPixels, origin)
     trg = np.array(trg, dtype=np.float32)
     if trg.ndim == 1:

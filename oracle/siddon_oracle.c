@@ -37,6 +37,14 @@
 #define RTOL_F 1e-5f
 #define RTOL_D 1e-5
 
+/* ORACLE_PLANT selects a deliberately wrong variant, compiled only by
+ * tests/test_oracle_siddon.py to show the golden vectors catch it:
+ *   1 no end trimming     2 plain floor/ceil index bounds (no snapping)
+ *   3 float32 list always 4 round-half-away midpoint index */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 static int cmp_double(const void *a, const void *b) {
     double x = *(const double *)a, y = *(const double *)b;
     /* numpy sorts NaN last */
@@ -152,10 +160,11 @@ int64_t oracle_siddons(int64_t nrays, const float *src, const float *trg,
             float vNn = ((s[a] + amin * dST[a]) - p0[a]) / d[a];
             double r0p = rint(v0p), r0n = rint(v0n);
             float rNp = rintf(vNp), rNn = rintf(vNn);
-            int32_t i0p = to_i32(isclose_d(v0p, r0p, epsilon) ? r0p : floor(v0p));
-            int32_t i0n = to_i32(isclose_d(v0n, r0n, epsilon) ? r0n : floor(v0n));
-            int32_t iNp = to_i32(isclose_f(vNp, rNp, eps_f) ? (double)rNp : (double)ceilf(vNp));
-            int32_t iNn = to_i32(isclose_f(vNn, rNn, eps_f) ? (double)rNn : (double)ceilf(vNn));
+            int snap = ORACLE_PLANT != 2;
+            int32_t i0p = to_i32(snap && isclose_d(v0p, r0p, epsilon) ? r0p : floor(v0p));
+            int32_t i0n = to_i32(snap && isclose_d(v0n, r0n, epsilon) ? r0n : floor(v0n));
+            int32_t iNp = to_i32(snap && isclose_f(vNp, rNp, eps_f) ? (double)rNp : (double)ceilf(vNp));
+            int32_t iNn = to_i32(snap && isclose_f(vNn, rNn, eps_f) ? (double)rNn : (double)ceilf(vNn));
             i0[a] = dST[a] > 0.0f ? i0p : i0n;
             iN[a] = dST[a] > 0.0f ? iNp : iNn;
         }
@@ -165,6 +174,7 @@ int64_t oracle_siddons(int64_t nrays, const float *src, const float *trg,
         if (hit) {
             bad[r] |= 2;
             int f64mode = (dST[0] == 0.0f) || (dST[1] == 0.0f) || (dST[2] == 0.0f);
+            if (ORACLE_PLANT == 3) f64mode = 0;
             int64_t L = 0;
             if (f64mode) { alist[L++] = amin; alist[L++] = amax; }
             else { flist[L++] = amin; flist[L++] = amax; }
@@ -184,7 +194,9 @@ int64_t oracle_siddons(int64_t nrays, const float *src, const float *trg,
             qsort(alist, (size_t)L, sizeof(double), cmp_double);
             int64_t b = 0, e = L;
             /* trim, siddon.py:789-793 (default atol=1e-8) */
-            if (f64mode) {
+            if (ORACLE_PLANT == 1) {
+                /* no trimming */
+            } else if (f64mode) {
                 if (isclose_d(alist[0], alist[1], 1e-8)) b = 1;
                 if (isclose_d(alist[e - 1], alist[e - 2], 1e-8)) e -= 1;
             } else {
@@ -200,7 +212,8 @@ int64_t oracle_siddons(int64_t nrays, const float *src, const float *trg,
                     double m = 0.5 * (A1 + A0);
                     for (int a = 0; a < 3; ++a) {
                         double pos = ((double)s[a] + m * (double)dST[a]) - (double)p0[a];
-                        idx[a] = to_i32(pos / (double)d[a]);
+                        idx[a] = ORACLE_PLANT == 4 ? to_i32(round(pos / (double)d[a]))
+                                                   : to_i32(pos / (double)d[a]);
                     }
                     len = (float)((double)distance * dA);
                 } else {
@@ -209,7 +222,8 @@ int64_t oracle_siddons(int64_t nrays, const float *src, const float *trg,
                     float m = 0.5f * (A1 + A0);
                     for (int a = 0; a < 3; ++a) {
                         float pos = (s[a] + m * dST[a]) - p0[a];
-                        idx[a] = to_i32((double)(pos / d[a]));
+                        idx[a] = ORACLE_PLANT == 4 ? to_i32(round((double)(pos / d[a])))
+                                                   : to_i32((double)(pos / d[a]));
                     }
                     len = distance * dA;
                 }

@@ -286,6 +286,20 @@ def main():
     out[key + "_view_src_sha"] = np.array(sha(s.astype(np.float32)))
     out[key + "_view_trg_sha"] = np.array(sha(t.astype(np.float32)))
 
+    # G: a ray whose midpoint index leaves the grid: flat mode raises
+    s = np.array([2.0, 2.0, -1.5])
+    t = np.array([1.99999994012882, -1.9565179063616578, 0.14530362421344067])
+    ind, fl, fo, raised = run_flat(siddons, s, t, (4, 4, 3), 1.0, 0.0)
+    x, y, z, ln, off, hit = run_unravel(siddons, s, t, (4, 4, 3), 1.0, 0.0)
+    out["oob_src"] = s
+    out["oob_trg"] = t
+    out["oob_grid"] = np.array([4, 4, 3, 1.0, 1.0, 1.0, 0, 0, 0], float)
+    out["oob_raised"] = np.array(raised)
+    out["oob_u"] = np.concatenate([x, y, z]).astype(np.int32)
+    out["oob_len"] = ln
+    out["oob_off"] = off
+    out["oob_hit"] = hit
+
     np.savez_compressed(args.out, **out)
     print(f"wrote {args.out} ({os.path.getsize(args.out)/1e6:.2f} MB) in "
           f"{time.time()-t0:.1f}s")

@@ -1,0 +1,99 @@
+"""ctypes binding of libvir_amd.so (include/vir_amd.h).
+
+The library is built in-tree by vir_amd.build.  There is no fallback: if the
+shared object is missing or a GPU is absent, calls fail loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvir_amd.so")
+
+VIR_OK = 0
+VIR_ACCUMULATE = 1
+VIR_SEQUENTIAL = 2
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int32
+U32 = ctypes.c_uint32
+F32 = ctypes.c_float
+
+
+class VirGrid(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("d", ctypes.c_float * 3),
+                ("o", ctypes.c_float * 3), ("eps", ctypes.c_double)]
+
+
+G = ctypes.POINTER(VirGrid)
+
+# name -> (restype, argtypes); the exported set of include/vir_amd.h
+SIGNATURES = {
+    "vir_version": (ctypes.c_int, []),
+    "vir_last_error": (ctypes.c_char_p, []),
+    "vir_siddon_count": (ctypes.c_int, [P, P, I64, G, P, P, P]),
+    "vir_siddon_fill": (ctypes.c_int, [P, P, I64, G, P, P, P, P, P, P, P]),
+    "vir_siddon_fp": (ctypes.c_int, [P, P, I64, G, P, P, U32, P, P]),
+    "vir_siddon_bp_workspace": (I64, [G, I32]),
+    "vir_siddon_bp": (ctypes.c_int, [P, P, I32, I32, I32, P, G, P, P, U32, P, P]),
+    "vir_csr_fp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
+    "vir_csr_transpose": (ctypes.c_int, [P, P, P, I64, I64, I64, P, P, P, P]),
+    "vir_csc_bp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
+    "vir_sirt_residual": (ctypes.c_int, [P, P, P, I64, P]),
+    "vir_sirt_update": (ctypes.c_int, [P, P, P, F32, I64, P]),
+}
+
+_lib = None
+
+
+class VirError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libvir_amd.so (once) with every signature declared."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VirError(f"{LIB_PATH} not built: run `python -m vir_amd.build`")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != VIR_OK:
+        msg = lib().vir_last_error()
+        msg = msg.decode() if msg else ""
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise VirError(f"{what}: {msg} (rc={rc})")
+
+
+def make_grid(n, d, o, eps):
+    g = VirGrid()
+    for a in range(3):
+        g.n[a] = int(n[a])
+        g.d[a] = float(np.float32(d[a]))
+        g.o[a] = float(np.float32(o[a]))
+    g.eps = float(eps)
+    return g
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
