@@ -17,6 +17,7 @@ constexpr int kRayThreads = 256;
 
 // ---------------------------------------------------------------- trace
 
+template <bool P2, bool ISO>
 __global__ __launch_bounds__(kRayThreads) void k_siddon_count(GridK g, const float* __restrict__ src,
                                                              const float* __restrict__ trg,
                                                              int64_t nrays, int32_t* __restrict__ counts,
@@ -32,7 +33,7 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_count(GridK g, const flo
         fl = 2;
         Walker W;
         walker_begin(g, R, W);
-        walk_entries(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float) {
+        walk_entries<P2, ISO>(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float) {
             ++cnt;
             if ((unsigned)ix >= (unsigned)g.n[0] || (unsigned)iy >= (unsigned)g.n[1] ||
                 (unsigned)iz >= (unsigned)g.n[2])
@@ -43,6 +44,7 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_count(GridK g, const flo
     flags[r] = fl;
 }
 
+template <bool P2, bool ISO>
 __global__ __launch_bounds__(kRayThreads) void k_siddon_fill(GridK g, const float* __restrict__ src,
                                                             const float* __restrict__ trg, int64_t nrays,
                                                             const int64_t* __restrict__ off,
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fill(GridK g, const floa
     const int64_t e_end = off[r + 1];
     Walker W;
     walker_begin(g, R, W);
-    walk_entries(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
+    walk_entries<P2, ISO, true>(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
         if (e >= e_end) return;  // count and fill walk identically; never past the row
         if (rav) {
             bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
@@ -76,6 +78,7 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fill(GridK g, const floa
 
 // ------------------------------------------------------------ forward
 
+template <bool P2, bool ISO>
 __global__ __launch_bounds__(kRayThreads) void k_siddon_fp(GridK g, const float* __restrict__ src,
                                                           const float* __restrict__ trg, int64_t nrays,
                                                           const float* __restrict__ vol,
@@ -86,27 +89,29 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fp(GridK g, const float*
     if (r >= nrays) return;
     Ray R;
     ray_setup(g, src + 3 * r, trg + 3 * r, R);
-    float sum = 0.0f, comp = 0.0f;
+    // float64 accumulation of the float32 products (the accumulator of
+    // projection/proj_numba.py:17), one-deep software pipeline: the gather
+    // issued for entry j is consumed while entry j+1 is being traced
+    double sum = 0.0;
     int bad = 0;
+    float pv = 0.0f, pl = 0.0f;
     if (R.hit) {
         Walker W;
         walker_begin(g, R, W);
-        const int nyz = g.n[1] * g.n[2];
-        walk_entries(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
-            if ((unsigned)ix >= (unsigned)g.n[0] || (unsigned)iy >= (unsigned)g.n[1] ||
-                (unsigned)iz >= (unsigned)g.n[2]) {
-                ++bad;
-                return;
-            }
-            float term = vol[(int64_t)ix * nyz + iy * g.n[2] + iz] * len;
-            // Kahan-Babuska (Neumaier) compensated sum
-            float t = sum + term;
-            if (fabsf(sum) >= fabsf(term)) comp += (sum - t) + term;
-            else comp += (term - t) + sum;
-            sum = t;
+        const unsigned nyz = (unsigned)(g.n[1] * g.n[2]);
+        walk_entries<P2, ISO>(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
+            bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
+                      (unsigned)iz < (unsigned)g.n[2];
+            bad += in ? 0 : 1;
+            unsigned vi = (unsigned)ix * nyz + (unsigned)iy * (unsigned)g.n[2] + (unsigned)iz;
+            float v = in ? vol[vi] : 0.0f;
+            sum += (double)(pv * pl);
+            pv = v;
+            pl = in ? len : 0.0f;
         });
     }
-    float res = sum + comp;
+    sum += (double)(pv * pl);
+    float res = (float)sum;
     if (accumulate) res += sino[r];
     sino[r] = res;
     if (bad && err) atomicAdd(err, bad);
@@ -120,10 +125,7 @@ struct ViewFoot {
     int su, sv;              // colour strides (rays of one colour never share a voxel)
 };
 
-template <int BX, int BY, int BZ>
-struct BrickCfg {
-    static constexpr int kVox = BX * BY * BZ;
-};
+constexpr float kGuard = 0.05f;  // guard band around a brick, in voxels
 
 // Project world point p with view matrix P (row-major 3x4).
 __device__ __forceinline__ void project(const float* P, float x, float y, float z, float& u, float& v,
@@ -133,181 +135,187 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
     w = P[8] * x + P[9] * y + P[10] * z + P[11];
 }
 
-template <int BX, int BY, int BZ, int NT>
-__global__ __launch_bounds__(NT) void k_siddon_bp_brick(GridK g, const float* __restrict__ src,
-                                                       const float* __restrict__ trg, int nviews, int nu,
-                                                       int nv, const float* __restrict__ proj,
-                                                       const float* __restrict__ sino,
-                                                       float* __restrict__ out, int views_per_chunk,
-                                                       int nbx, int nby, int nbz, int64_t out_stride) {
-    __shared__ float acc[BX * BY * BZ];
-    __shared__ ViewFoot foot;
+// Each wave owns one sub-brick of SBX x SBY x SBZ voxels with a private
+// on-chip accumulator; a workgroup of four waves covers a 2x2 block of
+// sub-bricks (neighbouring sub-bricks share rays through L1/L2).  Per view a
+// wave projects its sub-brick's guard box onto the detector, then walks the
+// footprint rays colour by colour: rays of one colour are at least one voxel
+// footprint apart, so the lanes of a wave never update the same voxel in the
+// same colour phase, and consecutive phases of one wave are ordered by the
+// in-order LDS pipeline -- no atomics and no workgroup barriers.
+template <int SBX, int SBY, int SBZ, bool P2, bool ISO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_siddon_bp_wave(GridK g, const float* __restrict__ src,
+                                                      const float* __restrict__ trg, int nviews, int nu,
+                                                      int nv, const float* __restrict__ proj,
+                                                      const float* __restrict__ sino,
+                                                      float* __restrict__ out, int views_per_chunk,
+                                                      int nbx, int nby, int nbz, int64_t out_stride) {
+    constexpr int kVox = SBX * SBY * SBZ;
+    __shared__ float acc_all[4 * kVox];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float* acc = acc_all + wave * kVox;
 
-    const int nbricks = nbx * nby * nbz;
-    const int brick = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int bz = brick % nbz;
-    const int by = (brick / nbz) % nby;
-    const int bx = brick / (nbz * nby);
-    const int b0[3] = {bx * BX, by * BY, bz * BZ};
-    const int b1[3] = {min(b0[0] + BX, g.n[0]), min(b0[1] + BY, g.n[1]), min(b0[2] + BZ, g.n[2])};
-    const int tid = threadIdx.x;
-    (void)nbricks;
+    // nbx/nby/nbz count workgroup regions (2*SBX x 2*SBY x SBZ)
+    const int region = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int rz = region % nbz;
+    const int ry = (region / nbz) % nby;
+    const int rx = region / (nbz * nby);
+    const int b0[3] = {rx * 2 * SBX + (wave & 1) * SBX, ry * 2 * SBY + (wave >> 1) * SBY, rz * SBZ};
+    const int b1[3] = {min(b0[0] + SBX, g.n[0]), min(b0[1] + SBY, g.n[1]), min(b0[2] + SBZ, g.n[2])};
+    const bool live = b0[0] < g.n[0] && b0[1] < g.n[1];
 
-    for (int i = tid; i < BX * BY * BZ; i += NT) acc[i] = 0.0f;
+    for (int i = lane; i < kVox; i += 64) acc[i] = 0.0f;
 
-    // guard box: the brick grown by one voxel (virtual planes past the grid
-    // edge are allowed) -- every entry whose index rounds into the brick lies
-    // geometrically inside it.
+    // guard box: the sub-brick grown by kGuard voxels.  An entry whose
+    // computed index falls in the sub-brick has its true midpoint within
+    // ~1e-3 voxel of it (f32 rounding of the midpoint position), so every
+    // such entry belongs to a segment that meets the guard box.
     float gb_lo[3], gb_hi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        gb_lo[a] = plane_coord(g, a, b0[a] - 1);
-        gb_hi[a] = plane_coord(g, a, b1[a] + 1);
+        gb_lo[a] = plane_coord(g, a, b0[a]) - kGuard * g.d[a];
+        gb_hi[a] = plane_coord(g, a, b1[a]) + kGuard * g.d[a];
     }
 
     const int a_begin = blockIdx.y * views_per_chunk;
-    const int a_end = min(nviews, a_begin + views_per_chunk);
-    const int lane = tid & 63;
+    const int a_end = live ? min(nviews, a_begin + views_per_chunk) : a_begin;
 
     for (int va = a_begin; va < a_end; ++va) {
         const float* P = proj + 12 * va;
-        if (tid < 64) {
-            // lanes 0..63: the 4x4x4 lattice of corner-voxel corners
-            // (planes b0, b0+1, b1-1, b1 per axis); lanes 0..7 also project the
-            // 8 guard-box corners.
-            int li = (lane >> 4) & 3, lj = (lane >> 2) & 3, lk = lane & 3;
-            int pi = li < 2 ? b0[0] + li : b1[0] - 3 + li;
-            int pj = lj < 2 ? b0[1] + lj : b1[1] - 3 + lj;
-            int pk = lk < 2 ? b0[2] + lk : b1[2] - 3 + lk;
-            float u, v, w;
-            project(P, plane_coord(g, 0, pi), plane_coord(g, 1, pj), plane_coord(g, 2, pk), u, v, w);
-            bool ok = w > 0.0f;
-            u = u / w;
-            v = v / w;
-            // per-corner-voxel footprint: lanes sharing (li>>1, lj>>1, lk>>1)
-            // are reduced; group them with xor masks over bits (4, 2, 0)
-            float umin = u, umax = u, vmin = v, vmax = v;
-            int okall = ok ? 1 : 0;
-            const int masks[3] = {16, 4, 1};
+        // lanes: the 4x4x4 lattice of corner-voxel corners (planes b0, b0+1,
+        // b1-1, b1 per axis); lanes 0..7 also project the guard-box corners
+        int li = (lane >> 4) & 3, lj = (lane >> 2) & 3, lk = lane & 3;
+        int pi = li < 2 ? b0[0] + li : b1[0] - 3 + li;
+        int pj = lj < 2 ? b0[1] + lj : b1[1] - 3 + lj;
+        int pk = lk < 2 ? b0[2] + lk : b1[2] - 3 + lk;
+        float u, v, w;
+        project(P, plane_coord(g, 0, pi), plane_coord(g, 1, pj), plane_coord(g, 2, pk), u, v, w);
+        int okall = w > 0.0f ? 1 : 0;
+        u = u / w;
+        v = v / w;
+        float umin = u, umax = u, vmin = v, vmax = v;
+        const int masks[3] = {16, 4, 1};
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                umin = fminf(umin, __shfl_xor(umin, masks[m]));
-                umax = fmaxf(umax, __shfl_xor(umax, masks[m]));
-                vmin = fminf(vmin, __shfl_xor(vmin, masks[m]));
-                vmax = fmaxf(vmax, __shfl_xor(vmax, masks[m]));
-                okall &= __shfl_xor(okall, masks[m]);
-            }
-            float fu = umax - umin, fv = vmax - vmin;
-            const int masks2[3] = {32, 8, 2};
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                fu = fmaxf(fu, __shfl_xor(fu, masks2[m]));
-                fv = fmaxf(fv, __shfl_xor(fv, masks2[m]));
-                okall &= __shfl_xor(okall, masks2[m]);
-            }
-            // guard-box footprint from lanes 0..7
-            float gu = 0.f, gv = 0.f, gw = 0.f;
-            project(P, (lane & 4) ? gb_hi[0] : gb_lo[0], (lane & 2) ? gb_hi[1] : gb_lo[1],
-                    (lane & 1) ? gb_hi[2] : gb_lo[2], gu, gv, gw);
-            int gok = (lane >= 8) || (gw > 0.0f);
-            float guu = lane < 8 ? gu / gw : VIR_INF, gvv = lane < 8 ? gv / gw : VIR_INF;
-            float gumin = guu, gvmin = gvv;
-            float gumax = lane < 8 ? guu : -VIR_INF, gvmax = lane < 8 ? gvv : -VIR_INF;
-#pragma unroll
-            for (int m = 1; m < 8; m <<= 1) {
-                gumin = fminf(gumin, __shfl_xor(gumin, m));
-                gvmin = fminf(gvmin, __shfl_xor(gvmin, m));
-                gumax = fmaxf(gumax, __shfl_xor(gumax, m));
-                gvmax = fmaxf(gvmax, __shfl_xor(gvmax, m));
-                gok &= __shfl_xor(gok, m);
-            }
-            if (lane == 0) {
-                ViewFoot f;
-                const float margin = 0.05f;
-                if (!gok || !okall || !(fu < 1.0e6f) || !(fv < 1.0e6f)) {
-                    // degenerate projection: take the whole detector, and
-                    // process one ray per phase group by the widest stride
-                    f.iu0 = 0; f.iu1 = nu - 1; f.iv0 = 0; f.iv1 = nv - 1;
-                    f.su = nu; f.sv = nv;  // hosts validate geometry so this is never taken
-                } else {
-                    float a0 = ceilf(gumin - margin), a1 = floorf(gumax + margin);
-                    float c0 = ceilf(gvmin - margin), c1 = floorf(gvmax + margin);
-                    f.iu0 = (int)fmaxf(a0, 0.0f);
-                    f.iu1 = (int)fminf(a1, (float)(nu - 1));
-                    f.iv0 = (int)fmaxf(c0, 0.0f);
-                    f.iv1 = (int)fminf(c1, (float)(nv - 1));
-                    if (a1 < 0.0f || a0 > (float)(nu - 1)) f.iu1 = f.iu0 - 1;
-                    if (c1 < 0.0f || c0 > (float)(nv - 1)) f.iv1 = f.iv0 - 1;
-                    f.su = (int)floorf(fu * 1.05f + margin) + 1;
-                    f.sv = nv == 1 ? 1 : (int)floorf(fv * 1.05f + margin) + 1;
-                    // a stride wider than the footprint already puts every
-                    // column (row) in its own phase
-                    f.su = min(f.su, max(f.iu1 - f.iu0 + 1, 1));
-                    f.sv = min(f.sv, max(f.iv1 - f.iv0 + 1, 1));
-                }
-                foot = f;
-            }
+        for (int m = 0; m < 3; ++m) {
+            umin = fminf(umin, __shfl_xor(umin, masks[m]));
+            umax = fmaxf(umax, __shfl_xor(umax, masks[m]));
+            vmin = fminf(vmin, __shfl_xor(vmin, masks[m]));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, masks[m]));
+            okall &= __shfl_xor(okall, masks[m]);
         }
-        __syncthreads();
-        const ViewFoot f = foot;
+        float fu = umax - umin, fv = vmax - vmin;
+        const int masks2[3] = {32, 8, 2};
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            fu = fmaxf(fu, __shfl_xor(fu, masks2[m]));
+            fv = fmaxf(fv, __shfl_xor(fv, masks2[m]));
+            okall &= __shfl_xor(okall, masks2[m]);
+        }
+        float gu = 0.f, gv = 0.f, gw = 0.f;
+        project(P, (lane & 4) ? gb_hi[0] : gb_lo[0], (lane & 2) ? gb_hi[1] : gb_lo[1],
+                (lane & 1) ? gb_hi[2] : gb_lo[2], gu, gv, gw);
+        int gok = (lane >= 8) || (gw > 0.0f);
+        float guu = lane < 8 ? gu / gw : VIR_INF, gvv = lane < 8 ? gv / gw : VIR_INF;
+        float gumin = guu, gvmin = gvv;
+        float gumax = lane < 8 ? guu : -VIR_INF, gvmax = lane < 8 ? gvv : -VIR_INF;
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1) {
+            gumin = fminf(gumin, __shfl_xor(gumin, m));
+            gvmin = fminf(gvmin, __shfl_xor(gvmin, m));
+            gumax = fmaxf(gumax, __shfl_xor(gumax, m));
+            gvmax = fmaxf(gvmax, __shfl_xor(gvmax, m));
+            gok &= __shfl_xor(gok, m);
+        }
+        // lane 0 holds the wave's results; make them wave-uniform
+        gumin = __shfl(gumin, 0); gumax = __shfl(gumax, 0);
+        gvmin = __shfl(gvmin, 0); gvmax = __shfl(gvmax, 0);
+        gok = __shfl(gok, 0); okall = __shfl(okall, 0);
+        fu = __shfl(fu, 0); fv = __shfl(fv, 0);
+
+        ViewFoot f;
+        const float margin = 0.05f;
+        if (!gok || !okall || !(fu < 1.0e6f) || !(fv < 1.0e6f)) {
+            // degenerate projection (hosts validate geometry, so not taken)
+            f.iu0 = 0; f.iu1 = nu - 1; f.iv0 = 0; f.iv1 = nv - 1;
+            f.su = nu; f.sv = nv;
+        } else {
+            float a0 = ceilf(gumin - margin), a1 = floorf(gumax + margin);
+            float c0 = ceilf(gvmin - margin), c1 = floorf(gvmax + margin);
+            f.iu0 = (int)fmaxf(a0, 0.0f);
+            f.iu1 = (int)fminf(a1, (float)(nu - 1));
+            f.iv0 = (int)fmaxf(c0, 0.0f);
+            f.iv1 = (int)fminf(c1, (float)(nv - 1));
+            if (a1 < 0.0f || a0 > (float)(nu - 1)) f.iu1 = f.iu0 - 1;
+            if (c1 < 0.0f || c0 > (float)(nv - 1)) f.iv1 = f.iv0 - 1;
+            f.su = (int)floorf(fu * 1.05f + margin) + 1;
+            f.sv = nv == 1 ? 1 : (int)floorf(fv * 1.05f + margin) + 1;
+            // a stride wider than the footprint already puts every column
+            // (row) in its own phase
+            f.su = min(f.su, max(f.iu1 - f.iu0 + 1, 1));
+            f.sv = min(f.sv, max(f.iv1 - f.iv0 + 1, 1));
+        }
         const int nU = f.iu1 - f.iu0 + 1, nV = f.iv1 - f.iv0 + 1;
-        if (nU > 0 && nV > 0) {
-            for (int cu = 0; cu < f.su; ++cu) {
-                for (int cv = 0; cv < f.sv; ++cv) {
-                    int us = f.iu0 + ((cu - f.iu0 % f.su) + f.su) % f.su;
-                    int vs = f.iv0 + ((cv - f.iv0 % f.sv) + f.sv) % f.sv;
-                    int mU = us <= f.iu1 ? (f.iu1 - us) / f.su + 1 : 0;
-                    int mV = vs <= f.iv1 ? (f.iv1 - vs) / f.sv + 1 : 0;
-                    int M = mU * mV;
-                    for (int q = tid; q < M; q += NT) {
-                        int iu = us + (q / mV) * f.su;
-                        int iv = vs + (q % mV) * f.sv;
-                        int64_t ray = ((int64_t)va * nu + iu) * nv + iv;
-                        float wgt = sino[ray];
-                        if (wgt == 0.0f) continue;
-                        Ray R;
-                        ray_setup(g, src + 3 * ray, trg + 3 * ray, R);
-                        if (!R.hit) continue;
-                        // alpha interval of the ray inside the guard box
-                        float alo = -VIR_INF, ahi = VIR_INF;
-                        bool miss = false;
+        if (nU <= 0 || nV <= 0) continue;
+        for (int cu = 0; cu < f.su; ++cu) {
+            for (int cv = 0; cv < f.sv; ++cv) {
+                int us = f.iu0 + ((cu - f.iu0 % f.su) + f.su) % f.su;
+                int vs = f.iv0 + ((cv - f.iv0 % f.sv) + f.sv) % f.sv;
+                int mU = us <= f.iu1 ? (f.iu1 - us) / f.su + 1 : 0;
+                int mV = vs <= f.iv1 ? (f.iv1 - vs) / f.sv + 1 : 0;
+                int M = mU * mV;
+                for (int q = lane; q < M; q += 64) {
+                    int iu = us + (q / mV) * f.su;
+                    int iv = vs + (q % mV) * f.sv;
+                    int64_t ray = ((int64_t)va * nu + iu) * nv + iv;
+                    float wgt = sino[ray];
+                    if (wgt == 0.0f) continue;
+                    Ray R;
+                    ray_setup_core(g, src + 3 * ray, trg + 3 * ray, R);
+                    if (!R.hit) continue;
+                    // alpha interval of the ray inside the guard box
+                    float alo = -VIR_INF, ahi = VIR_INF;
+                    bool miss = false;
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            if (R.dST[a] != 0.0f) {
-                                float t1 = (gb_lo[a] - R.s[a]) / R.dST[a];
-                                float t2 = (gb_hi[a] - R.s[a]) / R.dST[a];
-                                alo = fmaxf(alo, fminf(t1, t2));
-                                ahi = fminf(ahi, fmaxf(t1, t2));
-                            } else if (R.s[a] < gb_lo[a] || R.s[a] > gb_hi[a]) {
-                                miss = true;
-                            }
+                    for (int a = 0; a < 3; ++a) {
+                        if (R.dST[a] != 0.0f) {
+                            float t1 = (gb_lo[a] - R.s[a]) * R.y[a];
+                            float t2 = (gb_hi[a] - R.s[a]) * R.y[a];
+                            alo = fmaxf(alo, fminf(t1, t2));
+                            ahi = fminf(ahi, fmaxf(t1, t2));
+                        } else if (R.s[a] < gb_lo[a] || R.s[a] > gb_hi[a]) {
+                            miss = true;
                         }
-                        if (miss || alo > ahi) continue;
-                        Walker W;
-                        walker_seek(g, R, W, alo);
-                        walk_entries(g, R, W, ahi, [&](int ix, int iy, int iz, float len) {
-                            int lx = ix - b0[0], ly = iy - b0[1], lz = iz - b0[2];
-                            if ((unsigned)lx < (unsigned)(b1[0] - b0[0]) &&
-                                (unsigned)ly < (unsigned)(b1[1] - b0[1]) &&
-                                (unsigned)lz < (unsigned)(b1[2] - b0[2])) {
-                                acc[(lx * BY + ly) * BZ + lz] += wgt * len;
-                            }
-                        });
                     }
-                    __syncthreads();
+                    if (miss || alo > ahi) continue;
+                    // exact plane slices only matter near the ray's ends
+                    if (!window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
+                    Walker W;
+                    walker_seek(g, R, W, alo);
+                    walk_entries<P2, ISO>(g, R, W, ahi, [&](int ix, int iy, int iz, float len) {
+                        int lx = ix - b0[0], ly = iy - b0[1], lz = iz - b0[2];
+                        if ((unsigned)lx < (unsigned)(b1[0] - b0[0]) && (unsigned)ly < (unsigned)(b1[1] - b0[1]) &&
+                            (unsigned)lz < (unsigned)(b1[2] - b0[2])) {
+                            acc[(lx * SBY + ly) * SBZ + lz] += wgt * len;
+                        }
+                    });
                 }
+                // next colour phase: keep the compiler from moving LDS accesses
+                // across it (the hardware keeps one wave's LDS accesses in order)
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
         }
-        __syncthreads();
     }
 
-    // write the brick (or this view-chunk's partial brick)
+    // write the sub-brick (or this view chunk's partial sub-brick)
+    if (!live) return;
     float* dst = out + (int64_t)blockIdx.y * out_stride;
     const int ex = b1[0] - b0[0], ey = b1[1] - b0[1], ez = b1[2] - b0[2];
-    for (int i = tid; i < ex * ey * ez; i += NT) {
+    for (int i = lane; i < ex * ey * ez; i += 64) {
         int lz = i % ez, ly = (i / ez) % ey, lx = i / (ez * ey);
         int64_t gi = ((int64_t)(b0[0] + lx) * g.n[1] + (b0[1] + ly)) * g.n[2] + (b0[2] + lz);
-        dst[gi] = acc[(lx * BY + ly) * BZ + lz];
+        dst[gi] = acc[(lx * SBY + ly) * SBZ + lz];
     }
 }
 
@@ -327,24 +335,26 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int6
     vol[i] = s + c;
 }
 
-// Brick shapes: 3D grids use 32x32x16 (64 KiB of f32 accumulators), flat
-// (nZ == 1) grids 32x32x1.
+// Sub-brick shapes (one per wave, four waves per workgroup): 3D grids use
+// 16x16x8 voxels (8 KiB of f32 accumulators per wave), flat (nZ == 1) grids
+// 16x16x1.  Views are split into chunks until there are enough workgroups to
+// fill the chip; chunk partial volumes are summed in a fixed order.
 struct BpPlan {
-    int kind;  // 0: 3D bricks, 1: 2D bricks
-    int bx, by, bz;
-    int nbx, nby, nbz;
+    int kind;  // 0: 3D sub-bricks, 1: 2D sub-bricks
+    int sbx, sby, sbz;
+    int nbx, nby, nbz;  // workgroup regions (2*sbx x 2*sby x sbz)
     int nchunks, views_per_chunk;
 };
 
 static BpPlan plan_bp(const GridK& g, int nviews) {
     BpPlan p;
-    if (g.n[2] == 1) { p.kind = 1; p.bx = 32; p.by = 32; p.bz = 1; }
-    else { p.kind = 0; p.bx = 32; p.by = 32; p.bz = 16; }
-    p.nbx = (g.n[0] + p.bx - 1) / p.bx;
-    p.nby = (g.n[1] + p.by - 1) / p.by;
-    p.nbz = (g.n[2] + p.bz - 1) / p.bz;
+    if (g.n[2] == 1) { p.kind = 1; p.sbx = 16; p.sby = 16; p.sbz = 1; }
+    else { p.kind = 0; p.sbx = 16; p.sby = 16; p.sbz = 8; }
+    p.nbx = (g.n[0] + 2 * p.sbx - 1) / (2 * p.sbx);
+    p.nby = (g.n[1] + 2 * p.sby - 1) / (2 * p.sby);
+    p.nbz = (g.n[2] + p.sbz - 1) / p.sbz;
     int64_t nb = (int64_t)p.nbx * p.nby * p.nbz;
-    int target = p.kind == 0 ? 1024 : 2048;
+    const int target = 1024;
     int nch = (int)std::min<int64_t>(nviews, std::max<int64_t>(1, (target + nb - 1) / nb));
     p.views_per_chunk = (nviews + nch - 1) / nch;
     p.nchunks = (nviews + p.views_per_chunk - 1) / p.views_per_chunk;
@@ -356,6 +366,22 @@ static BpPlan plan_bp(const GridK& g, int nviews) {
 using namespace vir;
 
 extern "C" {
+
+static bool all_pow2(const GridK& g) { return g.pow2[0] && g.pow2[1] && g.pow2[2]; }
+static bool isotropic(const GridK& g) {
+    return g.n[0] == g.n[1] && g.n[1] == g.n[2] && g.d[0] == g.d[1] && g.d[1] == g.d[2] &&
+           g.o[0] == g.o[1] && g.o[1] == g.o[2];
+}
+
+// Launch KERNEL<P2, ISO> with the template flags chosen from the grid.
+#define VIR_DISPATCH_RAY_KERNEL(KERNEL, G, NB, ...)                                                       \
+    do {                                                                                                    \
+        bool p2_ = all_pow2(G), iso_ = isotropic(G);                                                       \
+        if (p2_ && iso_) hipLaunchKernelGGL((KERNEL<true, true>), __VA_ARGS__);                             \
+        else if (p2_) hipLaunchKernelGGL((KERNEL<true, false>), __VA_ARGS__);                               \
+        else if (iso_) hipLaunchKernelGGL((KERNEL<false, true>), __VA_ARGS__);                              \
+        else hipLaunchKernelGGL((KERNEL<false, false>), __VA_ARGS__);                                       \
+    } while (0)
 
 static int check_rays(const float* src, const float* trg, int64_t nrays) {
     if (nrays < 0) return fail_arg("nrays < 0");
@@ -372,8 +398,8 @@ int vir_siddon_count(const float* src, const float* trg, int64_t nrays, const vi
     if (nrays == 0) return VIR_OK;
     if (!counts || !flags) return fail_arg("counts/flags is NULL");
     int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
-    hipLaunchKernelGGL(k_siddon_count, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g, src,
-                       trg, nrays, counts, flags);
+    VIR_DISPATCH_RAY_KERNEL(k_siddon_count, g, nb, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream),
+                            g, src, trg, nrays, counts, flags);
     return check_hip(hipGetLastError(), "vir_siddon_count launch");
 }
 
@@ -387,8 +413,8 @@ int vir_siddon_fill(const float* src, const float* trg, int64_t nrays, const vir
     if (nrays == 0) return VIR_OK;
     if (!offsets) return fail_arg("offsets is NULL");
     int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
-    hipLaunchKernelGGL(k_siddon_fill, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g, src,
-                       trg, nrays, offsets, ravel_idx, ix, iy, iz, len);
+    VIR_DISPATCH_RAY_KERNEL(k_siddon_fill, g, nb, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g,
+                            src, trg, nrays, offsets, ravel_idx, ix, iy, iz, len);
     return check_hip(hipGetLastError(), "vir_siddon_fill launch");
 }
 
@@ -401,8 +427,9 @@ int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_g
     if (nrays == 0) return VIR_OK;
     if (!vol || !sino) return fail_arg("vol/sino is NULL");
     int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
-    hipLaunchKernelGGL(k_siddon_fp, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g, src, trg,
-                       nrays, vol, sino, (flags & VIR_ACCUMULATE) ? 1 : 0, err_count);
+    int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
+    VIR_DISPATCH_RAY_KERNEL(k_siddon_fp, g, nb, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g,
+                            src, trg, nrays, vol, sino, acc, err_count);
     return check_hip(hipGetLastError(), "vir_siddon_fp launch");
 }
 
@@ -439,13 +466,20 @@ int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu
     }
     int nb = p.nbx * p.nby * p.nbz;
     dim3 grd((unsigned)nb, (unsigned)p.nchunks);
-    if (p.kind == 0) {
-        hipLaunchKernelGGL((k_siddon_bp_brick<32, 32, 16, 256>), grd, dim3(256), 0, st, g, src, trg, nviews, nu,
-                           nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox);
-    } else {
-        hipLaunchKernelGGL((k_siddon_bp_brick<32, 32, 1, 64>), grd, dim3(64), 0, st, g, src, trg, nviews, nu,
-                           nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox);
-    }
+    const bool p2 = all_pow2(g), iso = isotropic(g);
+#define VIR_BP_LAUNCH(SX, SY, SZ)                                                                             \
+    do {                                                                                                        \
+        if (p2 && iso) hipLaunchKernelGGL((k_siddon_bp_wave<SX, SY, SZ, true, true>), grd, dim3(256), 0, st,      \
+                                          VIR_BP_ARGS);                                                         \
+        else if (p2) hipLaunchKernelGGL((k_siddon_bp_wave<SX, SY, SZ, true, false>), grd, dim3(256), 0, st,       \
+                                        VIR_BP_ARGS);                                                           \
+        else hipLaunchKernelGGL((k_siddon_bp_wave<SX, SY, SZ, false, false>), grd, dim3(256), 0, st, VIR_BP_ARGS); \
+    } while (0)
+#define VIR_BP_ARGS g, src, trg, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
+    if (p.kind == 0) VIR_BP_LAUNCH(16, 16, 8);
+    else VIR_BP_LAUNCH(16, 16, 1);
+#undef VIR_BP_ARGS
+#undef VIR_BP_LAUNCH
     rc = check_hip(hipGetLastError(), "vir_siddon_bp launch");
     if (rc) return rc;
     if (!direct) {

@@ -25,26 +25,57 @@ namespace vir {
 
 #define VIR_INF __builtin_huge_valf()
 
+// Per-axis ray and walker state lives in vector registers: clang vector types
+// keep dynamically selected lanes as register selects instead of letting the
+// optimiser turn them into indexed (scratch) memory accesses.
+typedef float f3v __attribute__((ext_vector_type(3)));
+typedef int i3v __attribute__((ext_vector_type(3)));
+
 struct GridK {
     int n[3];
     float d[3];
     float o[3];
     float p0[3];      // first plane coordinate per axis
     float pN[3];      // last plane coordinate per axis
-    float inv_d[3];   // 1/d when d is a power of two
-    int pow2[3];      // division by d may be replaced by multiplication
+    float inv_d[3];   // RN(1/d)
+    int pow2[3];      // d is a power of two: x/d == x*inv_d exactly
     float eps_f;      // epsilon as numpy uses it against float32 operands
+    float inv_eps;    // RN(1/eps_f)
     double eps_d;     // epsilon against float64 operands
 };
+
+// Correctly rounded a/b from y = RN(1/b) (Markstein): q = RN(a*y),
+// r = a - q*b (exact with FMA), RN(q + r*y) == RN(a/b) provided nothing
+// underflows.  Outside the guard band the IEEE division is used instead.
+// Checked bit-for-bit against IEEE division on 7e8 random operand pairs.
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+    float q = a * y;
+    float r = __builtin_fmaf(-q, b, a);
+    float qq = __builtin_fmaf(r, y, q);
+    float aq = fabsf(q);
+    bool ok = (fabsf(a) >= 0x1p-96f && aq >= 0x1p-100f && aq <= 0x1p100f) || a == 0.0f;
+    if (__builtin_expect(!ok, 0)) {
+        // keep this a real (rarely taken) branch rather than a select
+        asm volatile("" ::: "memory");
+        qq = a / b;
+    }
+    return qq;
+}
+
+__device__ __forceinline__ float fmin_nn(float a, float b) { return b < a ? b : a; }
 
 // boundspace(n, d, c, float32) (core/core_funcs.py:62-81):
 // float32(linspace(-n,n,n+1)) * d / 2 + c, each op rounded to float32.
 // Valid for any integer k (virtual planes outside [0, n] are used as guards).
-__device__ __forceinline__ float plane_coord(const GridK& g, int a, int k) {
-    float lin = (float)(2 * k - g.n[a]);
-    float t = lin * g.d[a];
+__device__ __forceinline__ float plane_at(int n, float d, float o, int k) {
+    float lin = (float)(2 * k - n);
+    float t = lin * d;
     t = t * 0.5f;
-    return t + g.o[a];
+    return t + o;
+}
+
+__device__ __forceinline__ float plane_coord(const GridK& g, int a, int k) {
+    return plane_at(g.n[a], g.d[a], g.o[a], k);
 }
 
 __device__ __forceinline__ bool isclose_f(float x, float y, float atol) {
@@ -76,22 +107,26 @@ __device__ __forceinline__ void py_slice(int start, int stop, int L, int& s, int
 }
 
 struct Ray {
-    float s[3];
-    float dST[3];
+    f3v s;
+    f3v dST;
+    f3v y;              // RN(1/dST) (unused where dST == 0)
     float dist;
     float amin, amax;
-    int k0[3], k1[3];   // plane slice [k0, k1) per axis; empty when dST == 0
+    i3v k0, k1;         // plane slice [k0, k1) per axis; empty when dST == 0
     bool hit;           // passed the entry/exit gate (siddon.py:759-760)
     bool f64;           // merged alpha list is float64 for this ray
 };
 
-// Per-ray setup, siddon.py:705-751 and the gate at :759-760.
-__device__ __forceinline__ void ray_setup(const GridK& g, const float* __restrict__ src,
-                                          const float* __restrict__ trg, Ray& R) {
+// Per-ray setup without the plane slices: endpoints, direction, length,
+// entry/exit alphas and the gate (siddon.py:705-736, 759-760).  The slices are
+// left at the full plane range [0, n+1).
+__device__ __forceinline__ void ray_setup_core(const GridK& g, const float* __restrict__ src,
+                                               const float* __restrict__ trg, Ray& R) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         R.s[a] = src[a];
         R.dST[a] = trg[a] - R.s[a];
+        R.y[a] = 1.0f / R.dST[a];
     }
     float sq0 = R.dST[0] * R.dST[0];
     float sq1 = R.dST[1] * R.dST[1];
@@ -100,11 +135,15 @@ __device__ __forceinline__ void ray_setup(const GridK& g, const float* __restric
 
     float amin = 0.0f, amax = 1.0f;
     bool nmin = false, nmax = false;
+    R.f64 = false;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float den = R.dST[a] != 0.0f ? R.dST[a] : g.eps_f;
-        float al0 = (g.p0[a] - R.s[a]) / den;
-        float alN = (g.pN[a] - R.s[a]) / den;
+        bool z = R.dST[a] == 0.0f;
+        R.f64 |= z;
+        float den = z ? g.eps_f : R.dST[a];
+        float iden = z ? g.inv_eps : R.y[a];
+        float al0 = div_rn(g.p0[a] - R.s[a], den, iden);
+        float alN = div_rn(g.pN[a] - R.s[a], den, iden);
         float mn = 0.0f, mx = 1.0f;
         if (al0 < alN) { mn = al0; mx = alN; }
         else if (al0 > alN) { mn = alN; mx = al0; }
@@ -112,25 +151,30 @@ __device__ __forceinline__ void ray_setup(const GridK& g, const float* __restric
         nmax |= __builtin_isnan(mx);
         amin = mn > amin ? mn : amin;
         amax = mx < amax ? mx : amax;
+        R.k0[a] = 0;
+        R.k1[a] = z ? 0 : g.n[a] + 1;
     }
     if (nmin) amin = __builtin_nanf("");
     if (nmax) amax = __builtin_nanf("");
     R.amin = amin;
     R.amax = amax;
+    R.hit = (amax > amin) && !isclose_f(amax, amin, g.eps_f);
+}
 
-    R.f64 = false;
+// The snapped plane-index bounds i0/iN of siddon.py:740-751 as python slice
+// bounds [k0, k1) of the plane arrays.
+__device__ __forceinline__ void ray_setup_slices(const GridK& g, Ray& R) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float dv = R.dST[a];
-        R.f64 |= (dv == 0.0f);
         // i0: positive direction uses alpha_min, otherwise alpha_max
-        float al_lo = dv > 0.0f ? amin : amax;
-        float al_hi = dv > 0.0f ? amax : amin;
-        float q0 = ((g.pN[a] - al_lo * dv) - R.s[a]) / g.d[a];
+        float al_lo = dv > 0.0f ? R.amin : R.amax;
+        float al_hi = dv > 0.0f ? R.amax : R.amin;
+        float q0 = div_rn((g.pN[a] - al_lo * dv) - R.s[a], g.d[a], g.inv_d[a]);
         double v0 = (double)(g.n[a] + 1) - (double)q0;
         double r0 = rint(v0);
         int i0 = to_i32_d(isclose_d(v0, r0, g.eps_d) ? r0 : floor(v0));
-        float vN = ((R.s[a] + al_hi * dv) - g.p0[a]) / g.d[a];
+        float vN = div_rn((R.s[a] + al_hi * dv) - g.p0[a], g.d[a], g.inv_d[a]);
         float rN = rintf(vN);
         int iN = to_i32_f(isclose_f(vN, rN, g.eps_f) ? rN : ceilf(vN));
         int ks, ke;
@@ -139,22 +183,55 @@ __device__ __forceinline__ void ray_setup(const GridK& g, const float* __restric
         R.k0[a] = ks;
         R.k1[a] = ke;
     }
-    R.hit = (amax > amin) && !isclose_f(amax, amin, g.eps_f);
+}
+
+// Full per-ray setup, siddon.py:705-751 and the gate at :759-760.
+__device__ __forceinline__ void ray_setup(const GridK& g, const float* __restrict__ src,
+                                          const float* __restrict__ trg, Ray& R) {
+    ray_setup_core(g, src, trg, R);
+    ray_setup_slices(g, R);
+}
+
+// True when the alpha window [t0, t1] lies at least three planes inside the
+// ray's entry and exit on every axis it crosses.  The snapped slice bounds
+// differ from the geometric entry/exit planes by at most one plane, so a walk
+// confined to such a window meets the same elements with or without them.
+__device__ __forceinline__ bool window_interior(const GridK& g, const Ray& R, float t0, float t1) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (R.dST[a] == 0.0f) continue;
+        float ui = (R.s[a] + R.amin * R.dST[a] - g.p0[a]) * g.inv_d[a];
+        float uo = (R.s[a] + R.amax * R.dST[a] - g.p0[a]) * g.inv_d[a];
+        float w0 = (R.s[a] + t0 * R.dST[a] - g.p0[a]) * g.inv_d[a];
+        float w1 = (R.s[a] + t1 * R.dST[a] - g.p0[a]) * g.inv_d[a];
+        float rlo = fminf(ui, uo) + 3.0f, rhi = fmaxf(ui, uo) - 3.0f;
+        float wlo = fminf(w0, w1) - 1.0f, whi = fmaxf(w0, w1) + 1.0f;
+        ok = ok && (wlo >= rlo) && (whi <= rhi);
+    }
+    return ok;
 }
 
 __device__ __forceinline__ float plane_alpha(const GridK& g, const Ray& R, int a, int k) {
-    return (plane_coord(g, a, k) - R.s[a]) / R.dST[a];
+    return div_rn(plane_coord(g, a, k) - R.s[a], R.dST[a], R.y[a]);
+}
+
+template <class T>
+__device__ __forceinline__ T sel3(int a, T v0, T v1, T v2) {
+    return a == 0 ? v0 : (a == 1 ? v1 : v2);
 }
 
 // Streaming merge of {alpha_min, alpha_max} and the three plane streams in
 // ascending alpha order.  Equal values may be taken in any order: the list
 // is a multiset and zero-length pairs are dropped by the reference
 // (siddon.py:803-809), so the emitted entries do not depend on tie order.
+// Exhausted streams hold +inf: every plane alpha in a stream lies close to
+// [alpha_min, alpha_max] within [0, 1], so +inf never collides with a value.
 struct Walker {
-    int kn[3];     // next plane index in stream order
-    int kstop[3];  // one past the last plane in stream order
-    int kdir[3];   // +1 ascending plane index, -1 descending
-    float h[3];    // alpha of plane kn (valid while kn != kstop)
+    i3v kn;        // next plane index in stream order
+    i3v kstop;     // one past the last plane in stream order
+    i3v kdir;      // +1 ascending plane index, -1 descending
+    f3v h;         // alpha of plane kn, +inf once the stream is exhausted
     int spec;      // 0: alpha_min pending, 1: alpha_max pending, 2: none
     int remaining; // elements not yet consumed
     float prev;    // last consumed element
@@ -182,82 +259,118 @@ __device__ __forceinline__ void walker_begin(const GridK& g, const Ray& R, Walke
 }
 
 // Consume and return the smallest pending element.  Requires remaining > 0.
+// Branch-free in the stream choice: one division per step whichever axis
+// advances.  ISO: all three axes share (n, d, origin), so the plane
+// coordinate needs no per-axis selection.
+template <bool ISO>
 __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker& W) {
-    int src = -1;
-    float best = VIR_INF;
-    if (W.spec < 2) { best = W.spec == 0 ? R.amin : R.amax; src = 3; }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (W.kn[a] != W.kstop[a] && (src < 0 || W.h[a] < best)) { best = W.h[a]; src = a; }
-    }
-    if (src == 3) {
-        W.spec += 1;
-    } else {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (a == src) {
-                W.kn[a] += W.kdir[a];
-                W.h[a] = W.kn[a] != W.kstop[a] ? plane_alpha(g, R, a, W.kn[a]) : VIR_INF;
-            }
-        }
-    }
+    float h0 = W.h[0], h1 = W.h[1], h2 = W.h[2];
+    bool c1 = h1 < h0;
+    float hm = c1 ? h1 : h0;
+    bool c2 = h2 < hm;
+    hm = c2 ? h2 : hm;
+    int am = c2 ? 2 : (c1 ? 1 : 0);
+    float sv = W.spec == 0 ? R.amin : R.amax;
     W.remaining -= 1;
-    return best;
+    if (W.spec < 2 && !(hm < sv)) {
+        W.spec += 1;
+        return sv;
+    }
+    bool s0 = am == 0, s1 = am == 1, s2 = am == 2;
+    int kd = s2 ? W.kdir[2] : (s1 ? W.kdir[1] : W.kdir[0]);
+    int k = (s2 ? W.kn[2] : (s1 ? W.kn[1] : W.kn[0])) + kd;
+    int ks = s2 ? W.kstop[2] : (s1 ? W.kstop[1] : W.kstop[0]);
+    float pc;
+    if (ISO) {
+        pc = plane_at(g.n[0], g.d[0], g.o[0], k);
+    } else {
+        pc = plane_at(s2 ? g.n[2] : (s1 ? g.n[1] : g.n[0]), s2 ? g.d[2] : (s1 ? g.d[1] : g.d[0]),
+                      s2 ? g.o[2] : (s1 ? g.o[1] : g.o[0]), k);
+    }
+    float sa = s2 ? R.s[2] : (s1 ? R.s[1] : R.s[0]);
+    float da = s2 ? R.dST[2] : (s1 ? R.dST[1] : R.dST[0]);
+    float ya = s2 ? R.y[2] : (s1 ? R.y[1] : R.y[0]);
+    float hn = div_rn(pc - sa, da, ya);
+    hn = k != ks ? hn : VIR_INF;
+    W.kn[0] = s0 ? k : W.kn[0];
+    W.kn[1] = s1 ? k : W.kn[1];
+    W.kn[2] = s2 ? k : W.kn[2];
+    W.h[0] = s0 ? hn : h0;
+    W.h[1] = s1 ? hn : h1;
+    W.h[2] = s2 ? hn : h2;
+    return hm;
 }
 
 // Position all streams just after every element <= T (used to enter a ray
 // part-way, at the guard plane of a volume brick).  Exact: each candidate
 // plane's alpha is recomputed by the same division and compared to T.
 __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker& W, float T) {
-    walker_begin(g, R, W);
+    W.remaining = 2;
     int cnt = 0;
     float pv = -VIR_INF;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (W.kn[a] == W.kstop[a]) continue;
-        float pos = R.s[a] + T * R.dST[a];
-        float fk = floorf((pos - g.p0[a]) / g.d[a]);
         int lo = R.k0[a], hi = R.k1[a];
-        if (W.kdir[a] > 0) {
+        if (R.dST[a] == 0.0f || lo >= hi) {
+            W.kn[a] = 0; W.kstop[a] = 0; W.kdir[a] = 1; W.h[a] = VIR_INF;
+            continue;
+        }
+        float pos = R.s[a] + T * R.dST[a];
+        float fk = floorf((pos - g.p0[a]) * g.inv_d[a]);
+        int k;
+        float hk;
+        if (R.dST[a] > 0.0f) {
             // first k in [lo, hi) with alpha(k) > T
-            int k = fk < -1.0e9f ? lo : (fk > 1.0e9f ? hi : (int)fk + 1);
+            k = fk < -1.0e9f ? lo : (fk > 1.0e9f ? hi : (int)fk + 1);
             k = k < lo ? lo : (k > hi ? hi : k);
             while (k > lo && plane_alpha(g, R, a, k - 1) > T) --k;
-            while (k < hi && plane_alpha(g, R, a, k) <= T) ++k;
+            hk = k < hi ? plane_alpha(g, R, a, k) : VIR_INF;
+            while (k < hi && hk <= T) {
+                ++k;
+                hk = k < hi ? plane_alpha(g, R, a, k) : VIR_INF;
+            }
             int taken = k - lo;
             if (taken > 0) pv = fmaxf(pv, plane_alpha(g, R, a, k - 1));
             cnt += taken;
-            W.kn[a] = k;
+            W.kn[a] = k; W.kstop[a] = hi; W.kdir[a] = 1;
+            W.remaining += hi - k;
         } else {
             // stream runs hi-1 .. lo; next = largest k in [lo, hi) with alpha(k) > T
-            int k = fk < -1.0e9f ? lo - 1 : (fk > 1.0e9f ? hi - 1 : (int)fk);
+            k = fk < -1.0e9f ? lo - 1 : (fk > 1.0e9f ? hi - 1 : (int)fk);
             k = k < lo - 1 ? lo - 1 : (k > hi - 1 ? hi - 1 : k);
             while (k < hi - 1 && plane_alpha(g, R, a, k + 1) > T) ++k;
-            while (k >= lo && plane_alpha(g, R, a, k) <= T) --k;
+            hk = k >= lo ? plane_alpha(g, R, a, k) : VIR_INF;
+            while (k >= lo && hk <= T) {
+                --k;
+                hk = k >= lo ? plane_alpha(g, R, a, k) : VIR_INF;
+            }
             int taken = (hi - 1) - k;
             if (taken > 0) pv = fmaxf(pv, plane_alpha(g, R, a, k + 1));
             cnt += taken;
-            W.kn[a] = k;
+            W.kn[a] = k; W.kstop[a] = lo - 1; W.kdir[a] = -1;
+            W.remaining += k - (lo - 1);
         }
-        W.remaining -= 0;  // recomputed below
-        W.h[a] = W.kn[a] != W.kstop[a] ? plane_alpha(g, R, a, W.kn[a]) : VIR_INF;
+        W.h[a] = hk;
     }
     W.spec = 0;
     if (R.amin <= T) { W.spec = 1; cnt += 1; pv = fmaxf(pv, R.amin); }
     if (W.spec == 1 && R.amax <= T) { W.spec = 2; cnt += 1; pv = fmaxf(pv, R.amax); }
-    int rem = 2 - W.spec;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) rem += stream_len(W, a);
-    W.remaining = rem;
+    W.remaining -= W.spec;
     W.consumed = cnt > 2 ? 2 : cnt;
     W.prev = pv;
 }
 
 // One merged segment (A[i], A[i+1]) -> voxel index and length, as
 // siddon.py:796-809.  Returns false when the reference drops the entry
-// (length not > 0).  idx holds the truncated index per axis.
+// (length not > 0).  idx holds the truncated index per axis.  P2: every
+// voxel size is a power of two, so dividing by it is a multiplication.
+template <bool P2, bool EXACT>
 __device__ __forceinline__ bool segment(const GridK& g, const Ray& R, float a0, float a1,
                                         int idx[3], float& len) {
+    // idx: truncated index per axis.  EXACT maps values outside int32 to
+    // INT32_MIN as numpy does on x86-64 (needed only where raw indices are
+    // returned); otherwise they saturate, which is equally outside the grid
+    // (NaN cannot occur: rays with NaN endpoints fail the gate).
     if (R.f64) {
         double A0 = (double)a0, A1 = (double)a1;
         double dA = A1 - A0;
@@ -265,8 +378,8 @@ __device__ __forceinline__ bool segment(const GridK& g, const Ray& R, float a0, 
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             double pos = ((double)R.s[a] + m * (double)R.dST[a]) - (double)g.p0[a];
-            double v = g.pow2[a] ? pos * (double)g.inv_d[a] : pos / (double)g.d[a];
-            idx[a] = to_i32_d(v);
+            double v = P2 ? pos * (double)g.inv_d[a] : pos / (double)g.d[a];
+            idx[a] = EXACT ? to_i32_d(v) : (int)v;
         }
         len = (float)((double)R.dist * dA);
     } else {
@@ -275,8 +388,8 @@ __device__ __forceinline__ bool segment(const GridK& g, const Ray& R, float a0, 
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float pos = (R.s[a] + m * R.dST[a]) - g.p0[a];
-            float v = g.pow2[a] ? pos * g.inv_d[a] : pos / g.d[a];
-            idx[a] = to_i32_f(v);
+            float v = P2 ? pos * g.inv_d[a] : div_rn(pos, g.d[a], g.inv_d[a]);
+            idx[a] = EXACT ? to_i32_f(v) : (int)v;
         }
         len = R.dist * dA;
     }
@@ -291,22 +404,22 @@ __device__ __forceinline__ bool trim_close(const Ray& R, float x, float y) {
 // Walk the merged list from the current walker state and call
 // visit(ix, iy, iz, len) for every entry the reference would emit, in
 // order.  Stops once the left end of the next pair exceeds a_stop.
-template <class Visit>
+template <bool P2, bool ISO, bool EXACT = false, class Visit>
 __device__ __forceinline__ void walk_entries(const GridK& g, const Ray& R, Walker& W,
                                              float a_stop, Visit&& visit) {
     if (W.consumed == 0) {
         if (W.remaining == 0) return;
-        W.prev = walker_pop(g, R, W);
+        W.prev = walker_pop<ISO>(g, R, W);
         W.consumed = 1;
     }
     while (W.remaining > 0 && !(W.prev > a_stop)) {
-        float b = walker_pop(g, R, W);
+        float b = walker_pop<ISO>(g, R, W);
         bool skip = false;
         if (W.consumed == 1) skip = trim_close(R, W.prev, b);     // drop A[0]
         if (W.remaining == 0) skip |= trim_close(R, b, W.prev);   // drop A[-1]
         int idx[3];
         float len;
-        bool keep = segment(g, R, W.prev, b, idx, len);
+        bool keep = segment<P2, EXACT>(g, R, W.prev, b, idx, len);
         if (keep && !skip) visit(idx[0], idx[1], idx[2], len);
         W.prev = b;
         W.consumed = 2;

@@ -49,6 +49,7 @@ int make_grid(const vir_grid* gp, GridK& g) {
     }
     if (!(gp->eps >= 0.0) || !std::isfinite(gp->eps)) return fail_arg("epsilon must be finite, >= 0");
     g.eps_f = (float)gp->eps;
+    g.inv_eps = 1.0f / g.eps_f;
     g.eps_d = gp->eps;
     return VIR_OK;
 }
