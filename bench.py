@@ -37,6 +37,20 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(kernel):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated against a
+    device copy of known size), or None when not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "cone256_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        return (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]) * 1024.0
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -146,6 +160,7 @@ def main():
     fp_bytes = 4.0 * nnz + 4.0 * nrays + 24.0 * nrays
     bp_bytes = 4.0 * nnz + 4.0 * nvox
     dom = "bp" if bp_ms >= fp_ms else "fp"
+    dom_kernel = "k_siddon_bp_wave" if dom == "bp" else "k_siddon_fp"
     dom_bytes, dom_ms = (bp_bytes, bp_ms) if dom == "bp" else (fp_bytes, fp_ms)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     rays_per_s = nrays * args.steps / wall
@@ -166,9 +181,9 @@ def main():
                    "volume": [256, 256, 256], "voxel_mm": 0.5, "views": int(P.layout[0]),
                    "detector": [512, 384], "pitch_mm": 1.0, "sid_mm": 500, "sdd_mm": 1000,
                    "rays": nrays, "nnz": nnz, "fp_ms": fp_ms, "bp_ms": bp_ms},
-        "roofline": {"bound": "hbm", "kernel": "k_siddon_bp_brick" if dom == "bp" else "k_siddon_fp",
+        "roofline": {"bound": "hbm", "kernel": dom_kernel,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_kernel),
                      "alg_bytes": dom_bytes},
     }
     if not args.no_cpu:
