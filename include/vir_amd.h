@@ -70,7 +70,8 @@ int vir_siddon_fill(const float *src, const float *trg, int64_t nrays,
 /* On-the-fly forward projection sino[r] = sum_j vol[idx_j] * len_j over the
  * ray's Siddon entries, with no stored matrix.  Same result as
  * siddons(flat=True) followed by the CSR gather of
- * projection/proj_numba.py:11-20 / proj_cuda.py:14-27; compensated f32 sum.
+ * projection/proj_numba.py:11-20 / proj_cuda.py:14-27, accumulated in float64
+ * as proj_numba.py:17 does.
  * err_count (may be NULL) receives the number of out-of-grid entries. */
 int vir_siddon_fp(const float *src, const float *trg, int64_t nrays,
                   const vir_grid *grid, const float *vol, float *sino,
@@ -117,6 +118,48 @@ int vir_csr_transpose(const int32_t *ind, const float *len, const int64_t *off,
 int vir_csc_bp(const int64_t *colptr, const int32_t *row, const float *clen,
                int64_t nvox, const float *sino, float *vol, uint32_t flags,
                void *stream);
+
+/* ------------------------------------------------------- distance-driven */
+
+/* Geometry of the distance-driven projectors (sys_mat/dd.py).  Volume
+ * (nx, ny[, nz]) of cubic voxels d_pix centred at the origin; detector of nu
+ * (x nv) bins of pitch du (dv) with offsets su (sv) in bins, boundaries
+ * du*(j - nu/2 + su) (dd.py:699, 777, 856-857); na views with per-view
+ * cos/sin given as device arrays.  mode: 0 parallel (dd.py:651-725),
+ * 1 fan with a flat detector at DSD (dd.py:730-803), 2 fan with an
+ * equiangular detector of arc pitch du at radius DSD (no reference
+ * counterpart).  The cone projector (dd.py:815-888) ignores mode. */
+typedef struct vir_dd_geom {
+    int32_t nx, ny, nz;
+    int32_t nu, nv, na;
+    int32_t mode;
+    float d_pix, du, dv, su, sv, DSO, DSD;
+} vir_dd_geom;
+
+/* sino[a, u] = out_scale * sum_pixels img * overlap * ray_scale, one thread
+ * per bin (gather).  The reference's dd_fp_par_2d / dd_fp_fan_2d use
+ * out_scale = d_pix/du (dd.py:725, 803). */
+int vir_dd2d_fp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
+                const float *img, float *sino, float out_scale, uint32_t flags,
+                void *stream);
+
+/* Exact transpose of vir_dd2d_fp (same weights), one thread per pixel.  The
+ * reference's dd_bp_par_2d / dd_bp_fan_2d use out_scale = 1/(na*d_pix)
+ * (dd.py:721-722, 799-800); out_scale = d_pix/du gives the exact adjoint. */
+int vir_dd2d_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
+                const float *sino, float *img, float out_scale, uint32_t flags,
+                void *stream);
+
+/* Cone-beam DD (dd.py:284-468, 815-888): sino (na, nu, nv); reference
+ * scaling out_scale = d_pix^2/(du*dv). */
+int vir_ddcone_fp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
+                  const float *vol, float *sino, float out_scale, uint32_t flags,
+                  void *stream);
+
+/* Exact transpose of vir_ddcone_fp; reference scaling 1/(na*d_pix). */
+int vir_ddcone_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
+                  const float *sino, float *vol, float out_scale, uint32_t flags,
+                  void *stream);
 
 /* ------------------------------------------------------------- SIRT ops */
 

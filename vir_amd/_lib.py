@@ -29,6 +29,18 @@ class VirGrid(ctypes.Structure):
 
 G = ctypes.POINTER(VirGrid)
 
+
+class VirDDGeom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("nu", ctypes.c_int32), ("nv", ctypes.c_int32), ("na", ctypes.c_int32),
+                ("mode", ctypes.c_int32),
+                ("d_pix", ctypes.c_float), ("du", ctypes.c_float), ("dv", ctypes.c_float),
+                ("su", ctypes.c_float), ("sv", ctypes.c_float), ("DSO", ctypes.c_float),
+                ("DSD", ctypes.c_float)]
+
+
+DG = ctypes.POINTER(VirDDGeom)
+
 # name -> (restype, argtypes); the exported set of include/vir_amd.h
 SIGNATURES = {
     "vir_version": (ctypes.c_int, []),
@@ -41,6 +53,10 @@ SIGNATURES = {
     "vir_csr_fp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
     "vir_csr_transpose": (ctypes.c_int, [P, P, P, I64, I64, I64, P, P, P, P]),
     "vir_csc_bp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
+    "vir_dd2d_fp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
+    "vir_dd2d_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
+    "vir_ddcone_fp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
+    "vir_ddcone_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_sirt_residual": (ctypes.c_int, [P, P, P, I64, P]),
     "vir_sirt_update": (ctypes.c_int, [P, P, P, F32, I64, P]),
 }

@@ -1,0 +1,363 @@
+// Distance-driven projectors for gfx950: 2-D parallel, 2-D fan (flat or
+// equiangular detector) and 3-D cone, forward and exact-adjoint backward.
+//
+// Model (sys_mat/dd.py): per view the driving axis is x when |sin| >= |cos|
+// (else y); each pixel row along the driving axis, taken at its orthogonal
+// centre, is projected onto the detector boundary by boundary; a pixel
+// contributes img * overlap(projected pixel interval, bin interval) * scale,
+// scale = 1/|sin| (parallel) or r / ((DSO - o) |sin|) (fan/cone, r the
+// in-plane source distance of the row centre).  The cone variant multiplies
+// by the overlap of the slab's magnified z interval with the detector row.
+//
+// Both directions are gathers (no atomics): the forward kernel runs one
+// thread per detector bin and finds the pixels whose shadows meet it by
+// inverting the projection; the backward kernel runs one thread per pixel
+// (voxel) and visits the bins its shadow meets.  Both evaluate every weight
+// through the same functions, so they are exact transposes.
+#include <cmath>
+
+#include "vir_common.h"
+
+namespace vir {
+
+enum DDMode { DD_PAR = 0, DD_FAN_FLAT = 1, DD_FAN_ARC = 2 };
+
+struct DDGeo {
+    int nx, ny, nz;
+    int nu, nv, na;
+    float d;            // voxel size
+    float du, dv;       // detector pitch (arc length for the equiangular fan)
+    float su, sv;       // detector offsets in bins
+    float DSO, DSD;
+    float out_scale;    // applied to every output element
+};
+
+struct DDView {
+    bool xd;            // x is the driving axis
+    float pd, po;       // p = pd * drive + po * orth
+    float od, oo;       // o = od * drive + oo * orth (distance toward the source)
+    float rs;           // |sin| or |cos| of the driving direction
+};
+
+__device__ __forceinline__ DDView dd_view(float c, float s) {
+    DDView v;
+    v.xd = fabsf(s) >= fabsf(c);
+    if (v.xd) { v.pd = -s; v.po = c; v.od = c; v.oo = s; v.rs = fabsf(s); }
+    else { v.pd = c; v.po = -s; v.od = s; v.oo = c; v.rs = fabsf(c); }
+    return v;
+}
+
+// boundary k of n cells of width d centred at 0: d * (k - n/2) in f32
+__device__ __forceinline__ float cell_bnd(int n, float d, int k) {
+    return d * ((float)k - 0.5f * (float)n);
+}
+
+__device__ __forceinline__ float cell_cen(int n, float d, int k) {
+    return (cell_bnd(n, d, k) + cell_bnd(n, d, k + 1)) * 0.5f;
+}
+
+__device__ __forceinline__ float det_bnd(int n, float du, float su, int j) {
+    return du * (((float)j - 0.5f * (float)n) + su);
+}
+
+// projected position of the driving-axis boundary k of orthogonal line io
+template <int MODE>
+__device__ __forceinline__ float dd_bound(const DDGeo& G, const DDView& V, float drv_b, float orth_c) {
+    float p = V.pd * drv_b + V.po * orth_c;
+    if (MODE == DD_PAR) return p;
+    float o = V.od * drv_b + V.oo * orth_c;
+    if (MODE == DD_FAN_FLAT) return (G.DSD * p) / (G.DSO - o);
+    return G.DSD * atan2f(p, G.DSO - o);
+}
+
+// ray scale of orthogonal line io
+template <int MODE>
+__device__ __forceinline__ float dd_scale(const DDGeo& G, const DDView& V, float orth_c) {
+    if (MODE == DD_PAR) return 1.0f / V.rs;
+    float po = V.po * orth_c, oo = V.oo * orth_c;
+    float den = G.DSO - oo;
+    return sqrtf(den * den + po * po) / (den * V.rs);
+}
+
+// inverse of dd_bound along the driving axis: the drive coordinate whose
+// boundary projects to u (approximate; callers widen the index range)
+template <int MODE>
+__device__ __forceinline__ float dd_invert(const DDGeo& G, const DDView& V, float u, float orth_c) {
+    float po = V.po * orth_c, oo = V.oo * orth_c;
+    if (MODE == DD_PAR) return (u - po) / V.pd;
+    float t = MODE == DD_FAN_FLAT ? u / G.DSD : tanf(u / G.DSD);
+    return (t * (G.DSO - oo) - po) / (V.pd + t * V.od);
+}
+
+__device__ __forceinline__ float overlap(float a0, float a1, float b0, float b1) {
+    float lo = a0 < a1 ? a0 : a1, hi = a0 < a1 ? a1 : a0;
+    return fminf(hi, b1) - fmaxf(lo, b0);
+}
+
+// ---------------------------------------------------------------- 2-D
+
+// One thread per (view, bin).  img is (nx, ny) C-order.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dd2d_fp(DDGeo G, const float* __restrict__ cs,
+                                                const float* __restrict__ sn, const float* __restrict__ img,
+                                                float* __restrict__ sino, int accumulate) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)G.na * G.nu) return;
+    const int ia = (int)(t / G.nu), iu = (int)(t % G.nu);
+    const DDView V = dd_view(cs[ia], sn[ia]);
+    const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
+    const float ul = det_bnd(G.nu, G.du, G.su, iu), ur = det_bnd(G.nu, G.du, G.su, iu + 1);
+    const float inv_d = 1.0f / G.d;
+    double acc = 0.0;
+    for (int io = 0; io < no; ++io) {
+        const float oc = cell_cen(no, G.d, io);
+        float x0 = dd_invert<MODE>(G, V, ul, oc), x1 = dd_invert<MODE>(G, V, ur, oc);
+        float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd;
+        float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd;
+        int k0 = (int)fmaxf(floorf(k0f) - 2.0f, 0.0f);
+        int k1 = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
+        if (!(k0f < (float)nd + 2.0f) || !(k1f > -2.0f)) continue;
+        const float sc = dd_scale<MODE>(G, V, oc);
+        float pb = dd_bound<MODE>(G, V, cell_bnd(nd, G.d, k0), oc);
+        for (int k = k0; k <= k1; ++k) {
+            float pn = dd_bound<MODE>(G, V, cell_bnd(nd, G.d, k + 1), oc);
+            float ov = overlap(pb, pn, ul, ur);
+            if (ov > 0.0f) {
+                int ix = V.xd ? k : io, iy = V.xd ? io : k;
+                acc += (double)(img[(int64_t)ix * G.ny + iy] * (ov * sc));
+            }
+            pb = pn;
+        }
+    }
+    float r = (float)acc * G.out_scale;
+    if (accumulate) r += sino[t];
+    sino[t] = r;
+}
+
+// One thread per pixel.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dd2d_bp(DDGeo G, const float* __restrict__ cs,
+                                                const float* __restrict__ sn, const float* __restrict__ sino,
+                                                float* __restrict__ img, int accumulate) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)G.nx * G.ny) return;
+    const int ix = (int)(t / G.ny), iy = (int)(t % G.ny);
+    const float u0 = det_bnd(G.nu, G.du, G.su, 0);
+    const float inv_du = 1.0f / G.du;
+    double acc = 0.0;
+    for (int ia = 0; ia < G.na; ++ia) {
+        const DDView V = dd_view(cs[ia], sn[ia]);
+        const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
+        const int k = V.xd ? ix : iy, io = V.xd ? iy : ix;
+        const float oc = cell_cen(no, G.d, io);
+        const float pa = dd_bound<MODE>(G, V, cell_bnd(nd, G.d, k), oc);
+        const float pb = dd_bound<MODE>(G, V, cell_bnd(nd, G.d, k + 1), oc);
+        const float lo = fminf(pa, pb), hi = fmaxf(pa, pb);
+        int j0 = (int)fmaxf(floorf((lo - u0) * inv_du) - 1.0f, 0.0f);
+        int j1 = (int)fminf(floorf((hi - u0) * inv_du) + 1.0f, (float)(G.nu - 1));
+        if (!(hi > u0 - G.du) || !((lo - u0) * inv_du < (float)G.nu + 1.0f)) continue;
+        const float sc = dd_scale<MODE>(G, V, oc);
+        const float* row = sino + (int64_t)ia * G.nu;
+        for (int j = j0; j <= j1; ++j) {
+            float ov = overlap(pa, pb, det_bnd(G.nu, G.du, G.su, j), det_bnd(G.nu, G.du, G.su, j + 1));
+            if (ov > 0.0f) acc += (double)(row[j] * (ov * sc));
+        }
+    }
+    float r = (float)acc * G.out_scale;
+    if (accumulate) r += img[t];
+    img[t] = r;
+}
+
+// ---------------------------------------------------------------- cone
+
+// slab z interval magnified onto the detector at the line's centre
+__device__ __forceinline__ float cone_zmag(const DDGeo& G, const DDView& V, float orth_c) {
+    return G.DSD / (G.DSO - V.oo * orth_c);
+}
+
+// One thread per (view, u, v) bin; sino is (na, nu, nv), vol (nx, ny, nz).
+__global__ __launch_bounds__(256) void k_ddcone_fp(DDGeo G, const float* __restrict__ cs,
+                                                  const float* __restrict__ sn, const float* __restrict__ vol,
+                                                  float* __restrict__ sino, int accumulate) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)G.na * G.nu * G.nv) return;
+    const int iv = (int)(t % G.nv);
+    const int iu = (int)((t / G.nv) % G.nu);
+    const int ia = (int)(t / ((int64_t)G.nv * G.nu));
+    const DDView V = dd_view(cs[ia], sn[ia]);
+    const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
+    const float ul = det_bnd(G.nu, G.du, G.su, iu), ur = det_bnd(G.nu, G.du, G.su, iu + 1);
+    const float vl = det_bnd(G.nv, G.dv, G.sv, iv), vr = det_bnd(G.nv, G.dv, G.sv, iv + 1);
+    const float inv_d = 1.0f / G.d;
+    double acc = 0.0;
+    for (int io = 0; io < no; ++io) {
+        const float oc = cell_cen(no, G.d, io);
+        float x0 = dd_invert<DD_FAN_FLAT>(G, V, ul, oc), x1 = dd_invert<DD_FAN_FLAT>(G, V, ur, oc);
+        float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd;
+        float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd;
+        if (!(k0f < (float)nd + 2.0f) || !(k1f > -2.0f)) continue;
+        const float zm = cone_zmag(G, V, oc);
+        float z0f = vl / zm * inv_d + 0.5f * (float)G.nz, z1f = vr / zm * inv_d + 0.5f * (float)G.nz;
+        if (!(z0f < (float)G.nz + 2.0f) || !(z1f > -2.0f)) continue;
+        int k0 = (int)fmaxf(floorf(k0f) - 2.0f, 0.0f);
+        int k1 = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
+        int z0 = (int)fmaxf(floorf(z0f) - 2.0f, 0.0f);
+        int z1 = (int)fminf(floorf(z1f) + 2.0f, (float)(G.nz - 1));
+        const float sc = dd_scale<DD_FAN_FLAT>(G, V, oc);
+        float pb = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, k0), oc);
+        for (int k = k0; k <= k1; ++k) {
+            float pn = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, k + 1), oc);
+            float ovu = overlap(pb, pn, ul, ur);
+            pb = pn;
+            if (!(ovu > 0.0f)) continue;
+            int ix = V.xd ? k : io, iy = V.xd ? io : k;
+            const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
+            for (int z = z0; z <= z1; ++z) {
+                float ovv = overlap(zm * cell_bnd(G.nz, G.d, z), zm * cell_bnd(G.nz, G.d, z + 1), vl, vr);
+                if (ovv > 0.0f) acc += (double)(col[z] * ((ovu * ovv) * sc));
+            }
+        }
+    }
+    float r = (float)acc * G.out_scale;
+    if (accumulate) r += sino[t];
+    sino[t] = r;
+}
+
+// One thread per voxel.
+__global__ __launch_bounds__(256) void k_ddcone_bp(DDGeo G, const float* __restrict__ cs,
+                                                  const float* __restrict__ sn, const float* __restrict__ sino,
+                                                  float* __restrict__ vol, int accumulate) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)G.nx * G.ny * G.nz) return;
+    const int iz = (int)(t % G.nz);
+    const int iy = (int)((t / G.nz) % G.ny);
+    const int ix = (int)(t / ((int64_t)G.nz * G.ny));
+    const float u0 = det_bnd(G.nu, G.du, G.su, 0), v0 = det_bnd(G.nv, G.dv, G.sv, 0);
+    const float inv_du = 1.0f / G.du, inv_dv = 1.0f / G.dv;
+    const float zb0 = cell_bnd(G.nz, G.d, iz), zb1 = cell_bnd(G.nz, G.d, iz + 1);
+    double acc = 0.0;
+    for (int ia = 0; ia < G.na; ++ia) {
+        const DDView V = dd_view(cs[ia], sn[ia]);
+        const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
+        const int k = V.xd ? ix : iy, io = V.xd ? iy : ix;
+        const float oc = cell_cen(no, G.d, io);
+        const float pa = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, k), oc);
+        const float pb = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, k + 1), oc);
+        const float zm = cone_zmag(G, V, oc);
+        const float qa = zm * zb0, qb = zm * zb1;
+        const float lo = fminf(pa, pb), hi = fmaxf(pa, pb);
+        const float vlo = fminf(qa, qb), vhi = fmaxf(qa, qb);
+        if (!(hi > u0 - G.du) || !((lo - u0) * inv_du < (float)G.nu + 1.0f)) continue;
+        if (!(vhi > v0 - G.dv) || !((vlo - v0) * inv_dv < (float)G.nv + 1.0f)) continue;
+        int j0 = (int)fmaxf(floorf((lo - u0) * inv_du) - 1.0f, 0.0f);
+        int j1 = (int)fminf(floorf((hi - u0) * inv_du) + 1.0f, (float)(G.nu - 1));
+        int m0 = (int)fmaxf(floorf((vlo - v0) * inv_dv) - 1.0f, 0.0f);
+        int m1 = (int)fminf(floorf((vhi - v0) * inv_dv) + 1.0f, (float)(G.nv - 1));
+        const float sc = dd_scale<DD_FAN_FLAT>(G, V, oc);
+        const float* plane = sino + (int64_t)ia * G.nu * G.nv;
+        for (int j = j0; j <= j1; ++j) {
+            float ovu = overlap(pa, pb, det_bnd(G.nu, G.du, G.su, j), det_bnd(G.nu, G.du, G.su, j + 1));
+            if (!(ovu > 0.0f)) continue;
+            for (int m = m0; m <= m1; ++m) {
+                float ovv = overlap(qa, qb, det_bnd(G.nv, G.dv, G.sv, m), det_bnd(G.nv, G.dv, G.sv, m + 1));
+                if (ovv > 0.0f) acc += (double)(plane[(int64_t)j * G.nv + m] * ((ovu * ovv) * sc));
+            }
+        }
+    }
+    float r = (float)acc * G.out_scale;
+    if (accumulate) r += vol[t];
+    vol[t] = r;
+}
+
+static int dd_check(const vir_dd_geom* g, DDGeo& G, bool cone) {
+    if (!g) return fail_arg("geometry is NULL");
+    G.nx = g->nx; G.ny = g->ny; G.nz = cone ? g->nz : 1;
+    G.nu = g->nu; G.nv = cone ? g->nv : 1; G.na = g->na;
+    G.d = g->d_pix; G.du = g->du; G.dv = cone ? g->dv : 1.0f;
+    G.su = g->su; G.sv = cone ? g->sv : 0.0f;
+    G.DSO = g->DSO; G.DSD = g->DSD;
+    if (G.nx < 1 || G.ny < 1 || G.nz < 1 || G.nu < 1 || G.nv < 1 || G.na < 0) return fail_arg("bad sizes");
+    if (!(G.d > 0.0f) || !(G.du > 0.0f) || !(G.dv > 0.0f)) return fail_arg("spacings must be > 0");
+    if (g->mode != DD_PAR || cone) {
+        if (!(G.DSO > 0.0f) || !(G.DSD > 0.0f)) return fail_arg("DSO and DSD must be > 0");
+        // every pixel must lie strictly in front of the source (monotone
+        // projections): half-diagonal of the grid < DSO
+        float hx = 0.5f * G.nx * G.d, hy = 0.5f * G.ny * G.d;
+        if (!(sqrtf(hx * hx + hy * hy) < G.DSO)) return fail_arg("grid reaches the source (DSO too small)");
+    }
+    return VIR_OK;
+}
+
+}  // namespace vir
+
+using namespace vir;
+
+extern "C" {
+
+int vir_dd2d_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* img, float* sino,
+                float out_scale, uint32_t flags, void* stream) {
+    DDGeo G;
+    int rc = dd_check(geom, G, false);
+    if (rc) return rc;
+    if (geom->mode < 0 || geom->mode > 2) return fail_arg("mode must be 0 (par), 1 (fan flat) or 2 (fan arc)");
+    if (G.na == 0) return VIR_OK;
+    if (!cosv || !sinv || !img || !sino) return fail_arg("NULL pointer argument");
+    G.out_scale = out_scale;
+    int64_t n = (int64_t)G.na * G.nu;
+    dim3 grd((unsigned)((n + 255) / 256));
+    int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
+    hipStream_t st = as_stream(stream);
+    if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_fp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
+    else if (geom->mode == DD_FAN_FLAT)
+        hipLaunchKernelGGL(k_dd2d_fp<DD_FAN_FLAT>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
+    else hipLaunchKernelGGL(k_dd2d_fp<DD_FAN_ARC>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
+    return check_hip(hipGetLastError(), "vir_dd2d_fp launch");
+}
+
+int vir_dd2d_bp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* sino, float* img,
+                float out_scale, uint32_t flags, void* stream) {
+    DDGeo G;
+    int rc = dd_check(geom, G, false);
+    if (rc) return rc;
+    if (geom->mode < 0 || geom->mode > 2) return fail_arg("mode must be 0 (par), 1 (fan flat) or 2 (fan arc)");
+    if ((G.na > 0 && (!cosv || !sinv || !sino)) || !img) return fail_arg("NULL pointer argument");
+    G.out_scale = out_scale;
+    int64_t n = (int64_t)G.nx * G.ny;
+    dim3 grd((unsigned)((n + 255) / 256));
+    int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
+    hipStream_t st = as_stream(stream);
+    if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_bp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc);
+    else if (geom->mode == DD_FAN_FLAT)
+        hipLaunchKernelGGL(k_dd2d_bp<DD_FAN_FLAT>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc);
+    else hipLaunchKernelGGL(k_dd2d_bp<DD_FAN_ARC>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc);
+    return check_hip(hipGetLastError(), "vir_dd2d_bp launch");
+}
+
+int vir_ddcone_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* vol, float* sino,
+                  float out_scale, uint32_t flags, void* stream) {
+    DDGeo G;
+    int rc = dd_check(geom, G, true);
+    if (rc) return rc;
+    if (G.na == 0) return VIR_OK;
+    if (!cosv || !sinv || !vol || !sino) return fail_arg("NULL pointer argument");
+    G.out_scale = out_scale;
+    int64_t n = (int64_t)G.na * G.nu * G.nv;
+    hipLaunchKernelGGL(k_ddcone_fp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), G, cosv,
+                       sinv, vol, sino, (flags & VIR_ACCUMULATE) ? 1 : 0);
+    return check_hip(hipGetLastError(), "vir_ddcone_fp launch");
+}
+
+int vir_ddcone_bp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* sino, float* vol,
+                  float out_scale, uint32_t flags, void* stream) {
+    DDGeo G;
+    int rc = dd_check(geom, G, true);
+    if (rc) return rc;
+    if ((G.na > 0 && (!cosv || !sinv || !sino)) || !vol) return fail_arg("NULL pointer argument");
+    G.out_scale = out_scale;
+    int64_t n = (int64_t)G.nx * G.ny * G.nz;
+    hipLaunchKernelGGL(k_ddcone_bp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), G, cosv,
+                       sinv, sino, vol, (flags & VIR_ACCUMULATE) ? 1 : 0);
+    return check_hip(hipGetLastError(), "vir_ddcone_bp launch");
+}
+
+}  // extern "C"

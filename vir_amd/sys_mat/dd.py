@@ -1,0 +1,181 @@
+"""Distance-driven projectors on the GPU.
+
+Drop-ins for the reference's sys_mat/dd.py drivers, same arguments and
+output scaling:
+
+  dd_fp_par_2d(img, ang_arr, nu, du=1.0, su=0.0, d_pix=1.0, sample=1)     dd.py:651-661
+  dd_bp_par_2d(sino, ang_arr, img_shape, du=1.0, su=0.0, d_pix=1.0)       dd.py:666-672
+  dd_fp_fan_2d(img, ang_arr, nu, DSO, DSD, du=1.0, su=0.0, d_pix=1.0)     dd.py:730-736
+  dd_bp_fan_2d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, su=0.0, d_pix=1.0)   dd.py:739-745
+  dd_fp_cone_3d(img, ang_arr, nu, nv, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=1.0,
+                d_pix=1.0, sample=1)                                       dd.py:815-825
+  dd_bp_cone_3d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, dv=1.0, su=0.0,
+                sv=0.0, d_pix=1.0)                                         dd.py:829-835
+
+The reference's backprojectors are A^T scaled by du/(na*d_pix^2) (2-D) and
+du*dv/(na*d_pix^3) (cone); these functions keep that scaling.  DDProjector is
+the device-resident operator pair with the exact adjoint (bp = A^T), plus an
+equiangular-detector fan variant (detector="arc").
+
+numpy in -> numpy out; CUDA tensors in -> CUDA tensors out.
+"""
+import numpy as np
+import torch
+
+from .. import _lib
+from ..device import as_device_f32, require_gpu, stream
+
+_MODES = {"par": 0, "flat": 1, "arc": 2}
+
+
+def as_float32(*args):
+    """Scalar -> np.float32, array -> float32 array (dd.py:29-36)."""
+    out = []
+    for x in args:
+        if isinstance(x, torch.Tensor):
+            out.append(x)
+        elif np.isscalar(x):
+            out.append(np.float32(x))
+        else:
+            out.append(np.asarray(x, dtype=np.float32))
+    return out
+
+
+class DDProjector:
+    """Distance-driven forward projector A and its exact adjoint A^T.
+
+    geom: "par", "fan" or "cone"; detector: "flat" (the reference's) or
+    "arc" (equiangular fan, du = arc length at radius DSD).  Angles in
+    radians; cos/sin are evaluated in float32 like the reference
+    (np.cos of a float32 array)."""
+
+    def __init__(self, img_shape, ang_arr, nu, geom="par", nv=1, DSO=0.0, DSD=0.0, du=1.0,
+                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, detector="flat"):
+        require_gpu()
+        ang = np.asarray(ang_arr, dtype=np.float32).reshape(-1)
+        self.na = ang.size
+        self.geom = geom
+        self.img_shape = tuple(int(v) for v in img_shape)
+        self.cos = torch.from_numpy(np.cos(ang).astype(np.float32)).cuda()
+        self.sin = torch.from_numpy(np.sin(ang).astype(np.float32)).cuda()
+        g = _lib.VirDDGeom()
+        g.nx, g.ny = self.img_shape[0], self.img_shape[1]
+        g.nz = self.img_shape[2] if geom == "cone" else 1
+        g.nu, g.nv, g.na = int(nu), int(nv) if geom == "cone" else 1, self.na
+        if geom == "par":
+            g.mode = 0
+        elif geom in ("fan", "cone"):
+            g.mode = _MODES[detector] if geom == "fan" else 1
+        else:
+            raise ValueError(f"unknown geometry {geom!r}")
+        for k, v in dict(d_pix=d_pix, du=du, dv=dv, su=su, sv=sv, DSO=DSO, DSD=DSD).items():
+            setattr(g, k, float(np.float32(v)))
+        self.g = g
+        self.d_pix, self.du, self.dv = np.float32(d_pix), np.float32(du), np.float32(dv)
+        if geom == "cone":
+            self.sino_shape = (self.na, int(nu), int(nv))
+        else:
+            self.sino_shape = (self.na, int(nu))
+        # A includes the reference's forward scaling
+        if geom == "cone":
+            self.fp_scale = float(self.d_pix * self.d_pix / (self.du * self.dv))
+            self.ref_bp_scale = float(np.float32(1.0) / (np.float32(self.na) * self.d_pix))
+        else:
+            self.fp_scale = float(self.d_pix / self.du)
+            self.ref_bp_scale = float(np.float32(1.0) / (np.float32(self.na) * self.d_pix))
+
+    def _call(self, name, src, dst, scale, accumulate):
+        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        _lib.check(getattr(_lib.lib(), name)(self.g, _lib.ptr(self.cos), _lib.ptr(self.sin),
+                                             _lib.ptr(src), _lib.ptr(dst), scale, fl, stream()), name)
+
+    def fp(self, img, out=None, accumulate=False):
+        img = as_device_f32(img)
+        if img.numel() != int(np.prod(self.img_shape)):
+            raise ValueError("image size does not match")
+        if out is None:
+            out = torch.empty(self.sino_shape, dtype=torch.float32, device=img.device)
+        self._call("vir_ddcone_fp" if self.geom == "cone" else "vir_dd2d_fp", img, out,
+                   self.fp_scale, accumulate)
+        return out
+
+    def bp(self, sino, out=None, accumulate=False, reference_scale=False):
+        """A^T sino; reference_scale=True returns the reference's scaled
+        backprojection (dd.py:721-722, 799-800, 886-887)."""
+        sino = as_device_f32(sino)
+        if sino.numel() != int(np.prod(self.sino_shape)):
+            raise ValueError("sinogram size does not match")
+        if out is None:
+            out = torch.empty(self.img_shape, dtype=torch.float32, device=sino.device)
+        scale = self.ref_bp_scale if reference_scale else self.fp_scale
+        self._call("vir_ddcone_bp" if self.geom == "cone" else "vir_dd2d_bp", sino, out, scale,
+                   accumulate)
+        return out
+
+
+def _wrap(x, torch_in):
+    return x if torch_in else x.cpu().numpy()
+
+
+def _kron_sample(img, sample):
+    if sample == 1:
+        return img
+    if isinstance(img, torch.Tensor):
+        img = img.cpu().numpy()
+    return np.kron(img, np.ones([sample] * img.ndim)).astype(np.float32)
+
+
+def dd_fp_par_2d(img, ang_arr, nu, du=1.0, su=0.0, d_pix=1.0, sample=1):
+    torch_in = isinstance(img, torch.Tensor)
+    img, ang_arr, du, su, d_pix = as_float32(img, ang_arr, du, su, d_pix)
+    P = DDProjector(tuple(img.shape), ang_arr, nu, "par", du=du, su=su, d_pix=d_pix)
+    return _wrap(P.fp(img), torch_in)
+
+
+def dd_bp_par_2d(sino, ang_arr, img_shape, du=1.0, su=0.0, d_pix=1.0):
+    torch_in = isinstance(sino, torch.Tensor)
+    sino, ang_arr, du, su, d_pix = as_float32(sino, ang_arr, du, su, d_pix)
+    P = DDProjector(img_shape, ang_arr, sino.shape[1], "par", du=du, su=su, d_pix=d_pix)
+    return _wrap(P.bp(sino, reference_scale=True), torch_in)
+
+
+def dd_fp_fan_2d(img, ang_arr, nu, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, detector="flat"):
+    torch_in = isinstance(img, torch.Tensor)
+    img, ang_arr, DSO, DSD, du, su, d_pix = as_float32(img, ang_arr, DSO, DSD, du, su, d_pix)
+    P = DDProjector(tuple(img.shape), ang_arr, nu, "fan", DSO=DSO, DSD=DSD, du=du, su=su,
+                    d_pix=d_pix, detector=detector)
+    return _wrap(P.fp(img), torch_in)
+
+
+def dd_bp_fan_2d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, detector="flat"):
+    torch_in = isinstance(sino, torch.Tensor)
+    sino, ang_arr, DSO, DSD, du, su, d_pix = as_float32(sino, ang_arr, DSO, DSD, du, su, d_pix)
+    P = DDProjector(img_shape, ang_arr, sino.shape[1], "fan", DSO=DSO, DSD=DSD, du=du, su=su,
+                    d_pix=d_pix, detector=detector)
+    return _wrap(P.bp(sino, reference_scale=True), torch_in)
+
+
+def dd_fp_cone_3d(img, ang_arr, nu, nv, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=1.0, d_pix=1.0,
+                  sample=1):
+    """Cone DD forward projection.  The u buffer is cleared fully per slab;
+    the reference clears only the current footprint (dd.py:317), which adds
+    leftovers when the object touches the volume border (identical results
+    otherwise)."""
+    torch_in = isinstance(img, torch.Tensor)
+    img, ang_arr, DSO, DSD, du, dv, su, sv, d_pix = as_float32(img, ang_arr, DSO, DSD, du, dv, su,
+                                                               sv, d_pix)
+    if sample != 1:
+        img = _kron_sample(img, sample)
+        d_pix = np.float32(d_pix / sample)
+    P = DDProjector(tuple(img.shape), ang_arr, nu, "cone", nv=nv, DSO=DSO, DSD=DSD, du=du, dv=dv,
+                    su=su, sv=sv, d_pix=d_pix)
+    return _wrap(P.fp(img), torch_in)
+
+
+def dd_bp_cone_3d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=0.0, d_pix=1.0):
+    torch_in = isinstance(sino, torch.Tensor)
+    sino, ang_arr, DSO, DSD, du, dv, su, sv, d_pix = as_float32(sino, ang_arr, DSO, DSD, du, dv,
+                                                                su, sv, d_pix)
+    P = DDProjector(img_shape, ang_arr, sino.shape[1], "cone", nv=sino.shape[2], DSO=DSO, DSD=DSD,
+                    du=du, dv=dv, su=su, sv=sv, d_pix=d_pix)
+    return _wrap(P.bp(sino, reference_scale=True), torch_in)
