@@ -1,0 +1,167 @@
+"""SIRT on the GPU: the reference's simultaneous iterative reconstruction
+loop with device-resident state.
+
+Recurrence (sys_mat/sirt_dd.py:37-76, tests/Test_SIRT/sirt_test.py:21-40,
+tests/Test_SIRT/2d_sirt.py:18-35, processing/SIRT/sirt.py:14-44):
+
+    r = (b - A x) / W_sino     where W_sino != 0   (else r = A x)
+    u = A^T r / W_img          where W_img  != 0   (else u = A^T r)
+    x = x + gamma * u
+
+with W_sino = A 1 and W_img = A^T 1 (sirt_test.py:157-158).  The
+weighted-residual and update steps are one elementwise kernel each
+(vir_sirt_residual, vir_sirt_update); one iteration is four launches on one
+stream and can be captured once into a HIP graph and replayed.
+
+Any projector with fp(x, out=) / bp(y, out=) on CUDA tensors works:
+SiddonProjector, DDProjector, StoredMatrix (below).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .device import as_device, as_device_f32, require_gpu, stream
+
+
+class StoredMatrix:
+    """A stored system matrix (the siddons() CSR) as an operator pair:
+    CSR gather forward, voxel-major transpose gather backward."""
+
+    def __init__(self, sdlist, nPixels, ray_shape=None):
+        from .projection.proj import _to_csr, _device_csr, _is_flat
+        from .sys_mat.siddon import csr_transpose
+        require_gpu()
+        self.img_shape = tuple(int(v) for v in np.atleast_1d(nPixels))
+        nvox = int(np.prod(self.img_shape))
+        ravel = True
+        if not _is_flat(sdlist):
+            flat = np.asarray(sdlist, dtype=object).reshape(-1)
+            first = next((r for r in flat if r is not None), None)
+            ravel = first is None or len(first) == 2
+            if ray_shape is None:
+                ray_shape = np.asarray(sdlist, dtype=object).shape
+        self.ind, self.len, self.off = _device_csr(_to_csr(sdlist, self.img_shape, ravel))
+        self.nrays = self.off.numel() - 1
+        self.sino_shape = tuple(ray_shape) if ray_shape is not None else (self.nrays,)
+        self.nvox = nvox
+        self.colptr, self.row, self.clen = csr_transpose(self.ind, self.len, self.off, nvox)
+
+    def fp(self, x, out=None, accumulate=False):
+        x = as_device_f32(x)
+        if out is None:
+            out = torch.empty(self.sino_shape, dtype=torch.float32, device=x.device)
+        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        _lib.check(_lib.lib().vir_csr_fp(_lib.ptr(self.ind), _lib.ptr(self.len), _lib.ptr(self.off),
+                                         self.nrays, _lib.ptr(x), _lib.ptr(out), fl, stream()),
+                   "vir_csr_fp")
+        return out
+
+    def bp(self, y, out=None, accumulate=False):
+        y = as_device_f32(y)
+        if out is None:
+            out = torch.empty(self.img_shape, dtype=torch.float32, device=y.device)
+        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        _lib.check(_lib.lib().vir_csc_bp(_lib.ptr(self.colptr), _lib.ptr(self.row), _lib.ptr(self.clen),
+                                         self.nvox, _lib.ptr(y), _lib.ptr(out), fl, stream()),
+                   "vir_csc_bp")
+        return out
+
+
+def _shapes(P):
+    img = getattr(P, "img_shape", None) or getattr(P, "nPixels")
+    sino = getattr(P, "sino_shape", None) or getattr(P, "ray_shape")
+    return tuple(img), tuple(sino)
+
+
+class SIRT:
+    """Device-resident SIRT for one projector.
+
+    W_sino / W_img default to A 1 and A^T 1.  step() runs one iteration in
+    place on self.x; run(n) runs n; graph=True captures one iteration into a
+    HIP graph on first use and replays it."""
+
+    def __init__(self, P, b, x0=None, W_sino=None, W_img=None, gamma=1.0, graph=False):
+        require_gpu()
+        self.P = P
+        self.img_shape, self.sino_shape = _shapes(P)
+        self.b = as_device_f32(b).reshape(self.sino_shape).contiguous()
+        dev = self.b.device
+        if W_sino is None:
+            W_sino = P.fp(torch.ones(self.img_shape, dtype=torch.float32, device=dev))
+        if W_img is None:
+            W_img = P.bp(torch.ones(self.sino_shape, dtype=torch.float32, device=dev))
+        self.W_sino = as_device_f32(W_sino).reshape(self.sino_shape).contiguous()
+        self.W_img = as_device_f32(W_img).reshape(self.img_shape).contiguous()
+        self.x = (torch.zeros(self.img_shape, dtype=torch.float32, device=dev) if x0 is None
+                  else as_device_f32(x0).reshape(self.img_shape).clone())
+        self.r = torch.empty(self.sino_shape, dtype=torch.float32, device=dev)
+        self.u = torch.empty(self.img_shape, dtype=torch.float32, device=dev)
+        self.gamma = float(gamma)
+        self.use_graph = graph
+        self._graph = None
+        self.iterations = 0
+
+    def _iteration(self):
+        L = _lib.lib()
+        st = stream()
+        self.P.fp(self.x, out=self.r)
+        _lib.check(L.vir_sirt_residual(_lib.ptr(self.b), _lib.ptr(self.r), _lib.ptr(self.W_sino),
+                                       self.r.numel(), st), "vir_sirt_residual")
+        self.P.bp(self.r, out=self.u)
+        _lib.check(L.vir_sirt_update(_lib.ptr(self.x), _lib.ptr(self.u), _lib.ptr(self.W_img),
+                                     self.gamma, self.x.numel(), st), "vir_sirt_update")
+
+    def step(self):
+        if not self.use_graph:
+            self._iteration()
+        else:
+            if self._graph is None:
+                # warm up on a side stream, then capture one iteration; the
+                # warm-up iteration is undone by restoring x
+                x_saved = self.x.clone()
+                s = torch.cuda.Stream()
+                s.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(s):
+                    self._iteration()
+                torch.cuda.current_stream().wait_stream(s)
+                self.x.copy_(x_saved)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    self._iteration()
+                self._graph = g
+                # capture does not execute: run the first iteration now
+            self._graph.replay()
+        self.iterations += 1
+        return self.x
+
+    def run(self, n):
+        for _ in range(int(n)):
+            self.step()
+        return self.x
+
+
+def sirt(sino, sdlist, W_sino, W_img, nPixels, gamma=1, beta=0, iters=10):
+    """Drop-in for tests/Test_SIRT/sirt_test.py:21-40: x0 = 0, `iters`
+    updates, returns every iterate as float32 array nPixels + (iters,).
+    beta > 0 applies total-variation denoising after each update
+    (sirt_test.py:37-38)."""
+    nPixels = tuple(int(v) for v in np.atleast_1d(nPixels))
+    P = StoredMatrix(sdlist, nPixels, ray_shape=np.shape(sino))
+    S = SIRT(P, sino, W_sino=W_sino, W_img=W_img, gamma=gamma)
+    out = np.zeros(nPixels + (iters,), dtype=np.float32)
+    for i in range(iters):
+        S.step()
+        if beta > 0:
+            from .tv import denoise_tv_chambolle
+            S.x.copy_(denoise_tv_chambolle(S.x, weight=beta))
+        out[..., i] = S.x.cpu().numpy()
+    return out
+
+
+def sirt_iterates(P, sino, iters, gamma=1.0, x0=None, W_sino=None, W_img=None, first=0):
+    """Generic driver: returns the final iterate after `iters` updates of the
+    recurrence starting from x0 (x0 = 1e-5 and iters-1 updates reproduces
+    sys_mat/sirt_dd.py:46-72)."""
+    S = SIRT(P, sino, x0=x0, W_sino=W_sino, W_img=W_img, gamma=gamma)
+    S.run(iters)
+    return S.x
