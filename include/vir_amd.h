@@ -161,6 +161,37 @@ int vir_ddcone_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
                   const float *sino, float *vol, float out_scale, uint32_t flags,
                   void *stream);
 
+/* ------------------------------------------ separable footprint (pixel-driven) */
+
+/* Geometry of the pixel-driven separable-footprint projectors
+ * (sys_mat/pd.py).  mode: 0 parallel 2-D (pd.py:129-234), 1 fan 2-D with a
+ * flat detector (pd.py:237-350), 2 cone 3-D (pd.py:353-485), 3 parallel 3-D
+ * with detector rows along z (per-slice 2-D parallel footprints times the
+ * overlap of each slab with the detector rows; no reference counterpart).
+ * Volumes (nbatch, nx, ny[, nz]); sinograms (nbatch, na, nu[, nv]).  ang,
+ * cosv, sinv: per-view angle and its float32 cos/sin (device arrays). */
+typedef struct vir_sf_geom {
+    int32_t nx, ny, nz;
+    int32_t nu, nv, na;
+    int32_t nbatch;
+    int32_t mode;
+    float d_pix, du, dv, su, sv, DSO, DSD;
+} vir_sf_geom;
+
+/* sino = out_scale * sum over pixels of img * amplitude * overlap/du
+ * (* overlap_v/dv); one thread per bin.  The reference scales by d_pix
+ * (pd.py:234, 350, 485). */
+int vir_sf_fp(const vir_sf_geom *geom, const float *ang, const float *cosv,
+              const float *sinv, const float *img, float *sino, float out_scale,
+              uint32_t flags, void *stream);
+
+/* Exact transpose of vir_sf_fp, one thread per pixel/voxel.  The reference's
+ * backprojectors use out_scale = du/(na*d_pix) (2-D, pd.py:231-232, 347-348)
+ * and du*dv/(na*d_pix) (cone, pd.py:482-483). */
+int vir_sf_bp(const vir_sf_geom *geom, const float *ang, const float *cosv,
+              const float *sinv, const float *sino, float *img, float out_scale,
+              uint32_t flags, void *stream);
+
 /* ------------------------------------------------------------- SIRT ops */
 
 /* r[i] = W[i] != 0 ? (b[i] - r[i]) / W[i] : r[i]   (sys_mat/sirt_dd.py:23-26,
@@ -172,6 +203,29 @@ int vir_sirt_residual(const float *b, float *r, const float *W, int64_t n,
  * sirt_test.py:33-34) */
 int vir_sirt_update(float *x, const float *u, const float *W, float gamma,
                     int64_t n, void *stream);
+
+/* --------------------------------------------------- total variation */
+
+#define VIR_TV_WORK 2048  /* doubles in the work buffer of vir_tv */
+
+/* Smoothed isotropic TV of volumes x (nbatch, nx, ny, nz), z fastest:
+ *   value = weight * sum_i (sqrt(|D x|_i^2 + eps^2) - eps)
+ * with forward differences D (zero at the last index), and its gradient
+ * grad = weight * dTV/dx (overwritten, or added with VIR_ACCUMULATE).
+ * grad and value (one device double) may each be NULL; value needs `work`
+ * (VIR_TV_WORK device doubles).  The regulariser of the reference's
+ * TV-constrained reconstruction (tv.py:200-224 gradient/divergence pair,
+ * tests/Test_SIRT/sirt_test.py:37-38) in differentiable form. */
+int vir_tv(const float *x, int64_t nbatch, int32_t nx, int32_t ny, int32_t nz,
+           float eps, float weight, float *grad, double *value, double *work,
+           uint32_t flags, void *stream);
+
+/* Least-squares data term: r = a - b (r may be NULL) and
+ * value = 0.5 * sum (a - b)^2 (one device double, may be NULL; needs `work`
+ * of VIR_TV_WORK doubles), summed in double in a fixed order.  No reference
+ * counterpart: the data term of a differentiable reconstruction loss. */
+int vir_sqdist(const float *a, const float *b, float *r, int64_t n,
+               double *value, double *work, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,143 @@
+"""GPU separable-footprint projectors against the float64 oracle
+(oracle/pd_ref.py, a restatement of sys_mat/pd.py).
+
+Bar: relative L2 error <= 1e-5 for forward and reference-scaled
+backprojection; <Ax, y> = <x, A^T y> to 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import pd_ref
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _adjoint(P, shape_x, shape_y, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    x = torch.from_numpy(rng.normal(size=shape_x).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.normal(size=shape_y).astype(np.float32)).cuda()
+    lhs = float((P.fp(x).double() * y.double()).sum())
+    rhs = float((x.double() * P.bp(y).double()).sum())
+    return abs(lhs - rhs) / abs(lhs)
+
+
+def test_par_2d_dropins():
+    from vir_amd.sys_mat import pd
+    rng = np.random.default_rng(0)
+    nx, ny, nu, du, su, d = 24, 20, 40, 0.9, 0.25, 1.1
+    ang = np.linspace(0, np.pi, 30, endpoint=False)
+    img = rng.uniform(0, 1, (nx, ny)).astype(np.float32)
+    img[:5] = 0.0
+    W = pd_ref.matrix("par", (nx, ny), ang, nu, du=du, su=su, d_pix=d)
+    sino = pd.pd_fp_par_2d(img, ang, nu, du=du, su=su, d_pix=d)
+    assert isinstance(sino, np.ndarray) and sino.shape == (30, nu) and sino.dtype == np.float32
+    assert rel(sino, pd_ref.fp(W, img, d)) <= TOL
+    y = rng.normal(size=(30, nu)).astype(np.float32)
+    bp = pd.pd_bp_par_2d(y, ang, (nx, ny), du=du, su=su, d_pix=d)
+    assert bp.shape == (nx, ny)
+    assert rel(bp, pd_ref.bp_reference(W, y, 30, d, du)) <= TOL
+
+
+def test_fan_2d_dropins_and_zero_runs():
+    from vir_amd.sys_mat import pd
+    rng = np.random.default_rng(1)
+    nx, ny, nu, du, su, d = 22, 26, 48, 1.3, -0.3, 0.9
+    DSO, DSD = 60.0, 100.0
+    ang = np.linspace(0, 2 * np.pi, 36, endpoint=False)
+    img = rng.uniform(0.1, 1, (nx, ny)).astype(np.float32)
+    W = pd_ref.matrix("fan", (nx, ny), ang, nu, du=du, su=su, d_pix=d, DSO=DSO, DSD=DSD)
+    sino = pd.pd_fp_fan_2d(img, ang, nu, DSO, DSD, du=du, su=su, d_pix=d)
+    assert rel(sino, pd_ref.fp(W, img, d)) <= TOL
+    # on a strictly nonzero image the reference's zero-skip path never runs
+    loop = pd_ref.fan_fp_loop(img, ang, nu, DSO, DSD, du, su, d, stale_oy=True)
+    assert rel(sino, loop) <= TOL
+    y = rng.normal(size=(36, nu)).astype(np.float32)
+    bp = pd.pd_bp_fan_2d(y, ang, (nx, ny), DSO, DSD, du=du, su=su, d_pix=d)
+    assert rel(bp, pd_ref.bp_reference(W, y, 36, d, du)) <= TOL
+    # zero runs: every footprint from its own boundaries
+    img[:, :6] = 0.0
+    img[3:9, 10:14] = 0.0
+    sino = pd.pd_fp_fan_2d(img, ang, nu, DSO, DSD, du=du, su=su, d_pix=d)
+    fixed = pd_ref.fan_fp_loop(img, ang, nu, DSO, DSD, du, su, d, stale_oy=False)
+    assert rel(sino, fixed) <= TOL
+
+
+def test_cone_3d_dropins():
+    from vir_amd.sys_mat import pd
+    rng = np.random.default_rng(2)
+    shape, nu, nv = (16, 14, 10), 24, 16
+    du, dv, su, sv, d = 1.1, 0.8, 0.2, 0.3, 0.9
+    DSO, DSD = 50.0, 80.0
+    ang = np.linspace(0, 2 * np.pi, 12, endpoint=False)
+    img = rng.uniform(0, 1, shape).astype(np.float32)
+    W = pd_ref.matrix("cone", shape, ang, nu, du=du, su=su, d_pix=d, DSO=DSO, DSD=DSD, nv=nv,
+                      dv=dv, sv=sv)
+    sino = pd.pd_fp_cone_3d(img, ang, nu, nv, DSO, DSD, du=du, dv=dv, su=su, sv=sv, d_pix=d)
+    assert sino.shape == (12, nu, nv)
+    assert rel(sino, pd_ref.fp(W, img, d)) <= TOL
+    y = rng.normal(size=(12, nu, nv)).astype(np.float32)
+    bp = pd.pd_bp_cone_3d(y, ang, shape, DSO, DSD, du=du, dv=dv, su=su, sv=sv, d_pix=d)
+    assert rel(bp, pd_ref.bp_reference(W, y, 12, d, du, dv)) <= TOL
+
+
+def test_par3d_batched_matches_oracle_per_volume():
+    import torch
+    from vir_amd.sys_mat.pd import SFProjector
+    rng = np.random.default_rng(3)
+    shape, nu, nv, nb = (12, 13, 6), 20, 8, 3
+    ang = np.linspace(0, np.pi, 10, endpoint=False)
+    kw = dict(du=0.95, su=0.1, d_pix=1.0, nv=nv, dv=0.75, sv=0.25)
+    W = pd_ref.matrix("par3d", shape, ang, nu, **kw)
+    P = SFProjector(shape, ang, nu, "par3d", batch=nb, **kw)
+    vols = rng.uniform(0, 1, (nb,) + shape).astype(np.float32)
+    sino = P.fp(torch.from_numpy(vols).cuda()).cpu().numpy()
+    assert sino.shape == (nb, 10, nu, nv)
+    y = rng.normal(size=sino.shape).astype(np.float32)
+    bp = P.bp(torch.from_numpy(y).cuda()).cpu().numpy()
+    for b in range(nb):
+        assert rel(sino[b], pd_ref.fp(W, vols[b], 1.0)) <= TOL
+        assert rel(bp[b], W.T @ y[b].reshape(-1).astype(np.float64)) <= TOL
+
+
+@pytest.mark.parametrize("geom", ["par", "fan", "cone", "par3d"])
+def test_adjoint(geom):
+    from vir_amd.sys_mat.pd import SFProjector
+    ang = np.linspace(0, 2 * np.pi, 17, endpoint=False)
+    if geom in ("par", "fan"):
+        P = SFProjector((31, 27), ang, 45, geom, DSO=70.0, DSD=110.0, du=1.2, su=0.3, d_pix=0.8)
+        sx, sy = (31, 27), (17, 45)
+    else:
+        P = SFProjector((20, 18, 9), ang, 30, geom, nv=14, DSO=70.0, DSD=110.0, du=1.1, dv=0.9,
+                        su=-0.2, sv=0.4, d_pix=0.7, batch=2)
+        sx, sy = (2, 20, 18, 9), (2, 17, 30, 14)
+    assert _adjoint(P, sx, sy, 7) <= TOL
+
+
+def test_accumulate_and_errors():
+    import torch
+    from vir_amd.sys_mat.pd import SFProjector
+    ang = np.linspace(0, np.pi, 8, endpoint=False)
+    P = SFProjector((10, 10), ang, 16, "par")
+    x = torch.rand(10, 10, device="cuda")
+    a = P.fp(x)
+    b = P.fp(x, out=a.clone(), accumulate=True)
+    assert torch.allclose(b, 2 * a, rtol=1e-6, atol=0)
+    # no views: an empty sinogram, and a zero backprojection
+    P0 = SFProjector((10, 10), np.zeros(0), 16, "par")
+    assert P0.fp(x).numel() == 0
+    assert float(P0.bp(torch.zeros(0, 16, device="cuda")).abs().sum()) == 0.0
+    with pytest.raises(ValueError):
+        SFProjector((40, 40), ang, 16, "fan", DSO=20.0, DSD=40.0).fp(torch.zeros(40, 40, device="cuda"))
+    with pytest.raises(ValueError):
+        # a detector reaching a 45 degree fan half-angle
+        SFProjector((10, 10), ang, 200, "fan", DSO=50.0, DSD=60.0).fp(torch.zeros(10, 10, device="cuda"))
+    with pytest.raises(ValueError):
+        P.fp(torch.zeros(9, 10, device="cuda"))

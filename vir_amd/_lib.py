@@ -41,6 +41,18 @@ class VirDDGeom(ctypes.Structure):
 
 DG = ctypes.POINTER(VirDDGeom)
 
+
+class VirSFGeom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("nu", ctypes.c_int32), ("nv", ctypes.c_int32), ("na", ctypes.c_int32),
+                ("nbatch", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("d_pix", ctypes.c_float), ("du", ctypes.c_float), ("dv", ctypes.c_float),
+                ("su", ctypes.c_float), ("sv", ctypes.c_float), ("DSO", ctypes.c_float),
+                ("DSD", ctypes.c_float)]
+
+
+SG = ctypes.POINTER(VirSFGeom)
+
 # name -> (restype, argtypes); the exported set of include/vir_amd.h
 SIGNATURES = {
     "vir_version": (ctypes.c_int, []),
@@ -57,7 +69,11 @@ SIGNATURES = {
     "vir_dd2d_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddcone_fp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddcone_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
+    "vir_sf_fp": (ctypes.c_int, [SG, P, P, P, P, P, F32, U32, P]),
+    "vir_sf_bp": (ctypes.c_int, [SG, P, P, P, P, P, F32, U32, P]),
     "vir_sirt_residual": (ctypes.c_int, [P, P, P, I64, P]),
+    "vir_tv": (ctypes.c_int, [P, I64, I32, I32, I32, F32, F32, P, P, P, U32, P]),
+    "vir_sqdist": (ctypes.c_int, [P, P, P, I64, P, P, P]),
     "vir_sirt_update": (ctypes.c_int, [P, P, P, F32, I64, P]),
 }
 

@@ -1,0 +1,307 @@
+// Pixel-driven separable-footprint projectors for gfx950 (sys_mat/pd.py):
+// 2-D parallel and fan, 3-D cone, and a batched 3-D parallel variant whose
+// detector rows run along z.
+//
+// Footprint model: each pixel's four corners are projected onto the detector
+// and the pixel deposits over [min, max] of them with amplitude
+//   parallel : 1/(|sin| + |cos|)                           (pd.py:181)
+//   fan      : 1/(|cos(a+g)| + |sin(a+g)|), g = atan(u_c/DSD) (pd.py:292-298)
+//   cone     : 1/(|sin| + |cos|), u from the four parallel-projected
+//              corners times the centre's fan magnification, v from the
+//              slab bounds times magnification/cos(g)   (pd.py:423-451)
+// weight per bin = overlap/du (* overlap_v/dv); the forward output is
+// scaled by d_pix (pd.py:234, 350, 485).
+//
+// Forward: one thread per (batch, view, bin[, row]) gathers the pixels whose
+// footprints meet the bin (centre projection inverted along the driving axis,
+// widened by the largest footprint half-width).  Backward: one thread per
+// (batch, pixel[, slab]) visits the bins its footprint meets.  Both evaluate
+// every weight with the same functions: exact transposes, no atomics.
+#include <cmath>
+
+#include "vir_common.h"
+
+namespace vir {
+
+enum SFMode { SF_PAR2 = 0, SF_FAN2 = 1, SF_CONE3 = 2, SF_PAR3 = 3 };
+
+struct SFGeo {
+    int nx, ny, nz, nu, nv, na, nb;
+    float d, du, dv, su, sv, DSO, DSD;
+    float out_scale;
+    float wmax;  // largest footprint half-width along u (detector units)
+};
+
+__device__ __forceinline__ float sf_bnd(int n, float d, int k) { return d * ((float)k - 0.5f * (float)n); }
+__device__ __forceinline__ float sf_det(int n, float du, float su, int j) {
+    return du * (((float)j - 0.5f * (float)n) + su);
+}
+
+struct SFFoot {
+    float u0, u1;   // footprint along u
+    float amp;      // amplitude
+    float cm;       // v magnification (cone) or 1
+};
+
+// Footprint of pixel (ix, iy) in view (c, s, ang).
+template <int MODE>
+__device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, float c, float s, float ang, int ix, int iy) {
+    const float xl = sf_bnd(G.nx, G.d, ix), xr = sf_bnd(G.nx, G.d, ix + 1);
+    const float yl = sf_bnd(G.ny, G.d, iy), yr = sf_bnd(G.ny, G.d, iy + 1);
+    const float pxl = -s * xl, pxr = -s * xr, pyl = c * yl, pyr = c * yr;
+    SFFoot f;
+    f.cm = 1.0f;
+    if (MODE == SF_PAR2 || MODE == SF_PAR3) {
+        float p0 = pyl + pxl, p1 = pyl + pxr, p2 = pyr + pxl, p3 = pyr + pxr;
+        f.u0 = fminf(fminf(p0, p1), fminf(p2, p3));
+        f.u1 = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
+        f.amp = 1.0f / (fabsf(s) + fabsf(c));
+        return f;
+    }
+    const float oxl = c * xl, oxr = c * xr, oyl = s * yl, oyr = s * yr;
+    const float pxc = (pxl + pxr) * 0.5f, oxc = (oxl + oxr) * 0.5f;
+    const float pyc = (pyl + pyr) * 0.5f, oyc = (oyl + oyr) * 0.5f;
+    const float den_c = G.DSO - (oxc + oyc);
+    const float uc = G.DSD * (pxc + pyc) / den_c;
+    const float gam = atanf(uc / G.DSD);
+    if (MODE == SF_FAN2) {
+        float p0 = G.DSD * (pxl + pyl) / (G.DSO - (oxl + oyl));
+        float p1 = G.DSD * (pxr + pyl) / (G.DSO - (oxr + oyl));
+        float p2 = G.DSD * (pxl + pyr) / (G.DSO - (oxl + oyr));
+        float p3 = G.DSD * (pxr + pyr) / (G.DSO - (oxr + oyr));
+        f.u0 = fminf(fminf(p0, p1), fminf(p2, p3));
+        f.u1 = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
+        float rx = cosf(ang + gam), ry = sinf(ang + gam);
+        f.amp = 1.0f / (fabsf(rx) + fabsf(ry));
+        return f;
+    }
+    // cone
+    const float fm = G.DSD / den_c;
+    float u0 = fm * (pxl + pyl), u1 = fm * (pxr + pyl), u2 = fm * (pxl + pyr), u3 = fm * (pxr + pyr);
+    f.u0 = fminf(fminf(u0, u1), fminf(u2, u3));
+    f.u1 = fmaxf(fmaxf(u0, u1), fmaxf(u2, u3));
+    f.cm = fm / cosf(gam);
+    f.amp = 1.0f / (fabsf(s) + fabsf(c));
+    return f;
+}
+
+__device__ __forceinline__ float ovl(float a0, float a1, float b0, float b1) {
+    return fminf(a1, b1) - fmaxf(a0, b0);
+}
+
+// centre projection along the driving axis, inverted: the drive coordinate
+// whose pixel centre projects to u on orthogonal line coordinate oc
+template <int MODE>
+__device__ __forceinline__ float sf_invert(const SFGeo& G, bool xd, float c, float s, float u, float oc) {
+    // p = pd*drive + po*orth, o = od*drive + oo*orth (o toward the source)
+    float pd = xd ? -s : c, po = xd ? c : -s;
+    float od = xd ? c : s, oo = xd ? s : c;
+    if (MODE == SF_PAR2 || MODE == SF_PAR3) return (u - po * oc) / pd;
+    float t = u / G.DSD;
+    return (t * (G.DSO - oo * oc) - po * oc) / (pd + t * od);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict__ ang,
+                                              const float* __restrict__ cs, const float* __restrict__ sn,
+                                              const float* __restrict__ img, float* __restrict__ sino,
+                                              int accumulate) {
+    const bool is3 = MODE == SF_CONE3 || MODE == SF_PAR3;
+    const int nv = is3 ? G.nv : 1;
+    const int64_t per_b = (int64_t)G.na * G.nu * nv;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per_b * G.nb) return;
+    const int b = (int)(t / per_b);
+    int64_t r = t % per_b;
+    const int iv = (int)(r % nv);
+    const int iu = (int)((r / nv) % G.nu);
+    const int ia = (int)(r / ((int64_t)nv * G.nu));
+    const float c = cs[ia], s = sn[ia], a = ang[ia];
+    const bool xd = fabsf(s) >= fabsf(c);
+    const int nd = xd ? G.nx : G.ny, no = xd ? G.ny : G.nx;
+    const float ul = sf_det(G.nu, G.du, G.su, iu), ur = sf_det(G.nu, G.du, G.su, iu + 1);
+    const float vl = sf_det(nv, G.dv, G.sv, iv), vr = sf_det(nv, G.dv, G.sv, iv + 1);
+    const float inv_d = 1.0f / G.d, inv_du = 1.0f / G.du, inv_dv = 1.0f / G.dv;
+    const int nz = is3 ? G.nz : 1;
+    const float* vol = img + (int64_t)b * G.nx * G.ny * nz;
+    double acc = 0.0;
+    for (int io = 0; io < no; ++io) {
+        const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
+        float x0 = sf_invert<MODE>(G, xd, c, s, ul - G.wmax, oc);
+        float x1 = sf_invert<MODE>(G, xd, c, s, ur + G.wmax, oc);
+        float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
+        float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
+        if (!(k0f < (float)nd + 1.0f) || !(k1f > -2.0f)) continue;
+        int k0 = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
+        int k1 = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
+        for (int k = k0; k <= k1; ++k) {
+            const int ix = xd ? k : io, iy = xd ? io : k;
+            const SFFoot f = sf_foot<MODE>(G, c, s, a, ix, iy);
+            const float ou = ovl(f.u0, f.u1, ul, ur);
+            if (!(ou > 0.0f)) continue;
+            const float wu = (ou * inv_du) * f.amp;
+            const float* col = vol + ((int64_t)ix * G.ny + iy) * nz;
+            if (!is3) {
+                acc += (double)(col[0] * wu);
+            } else {
+                // slabs whose magnified z interval meets [vl, vr]
+                float z0f = fminf(vl, vr) / f.cm * inv_d + 0.5f * (float)G.nz;
+                float z1f = fmaxf(vl, vr) / f.cm * inv_d + 0.5f * (float)G.nz;
+                int z0 = (int)fmaxf(floorf(z0f) - 1.0f, 0.0f);
+                int z1 = (int)fminf(floorf(z1f) + 1.0f, (float)(G.nz - 1));
+                for (int z = z0; z <= z1; ++z) {
+                    float ov = ovl(f.cm * sf_bnd(G.nz, G.d, z), f.cm * sf_bnd(G.nz, G.d, z + 1), vl, vr);
+                    if (ov > 0.0f) acc += (double)(col[z] * (wu * (ov * inv_dv)));
+                }
+            }
+        }
+    }
+    float res = (float)acc * G.out_scale;
+    if (accumulate) res += sino[t];
+    sino[t] = res;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sf_bp(SFGeo G, const float* __restrict__ ang,
+                                              const float* __restrict__ cs, const float* __restrict__ sn,
+                                              const float* __restrict__ sino, float* __restrict__ img,
+                                              int accumulate) {
+    const bool is3 = MODE == SF_CONE3 || MODE == SF_PAR3;
+    const int nz = is3 ? G.nz : 1, nv = is3 ? G.nv : 1;
+    const int64_t per_b = (int64_t)G.nx * G.ny * nz;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per_b * G.nb) return;
+    const int b = (int)(t / per_b);
+    int64_t r = t % per_b;
+    const int iz = (int)(r % nz);
+    const int iy = (int)((r / nz) % G.ny);
+    const int ix = (int)(r / ((int64_t)nz * G.ny));
+    const float u0 = sf_det(G.nu, G.du, G.su, 0), v0 = sf_det(nv, G.dv, G.sv, 0);
+    const float inv_du = 1.0f / G.du, inv_dv = 1.0f / G.dv;
+    const float zl = sf_bnd(G.nz, G.d, iz), zr = sf_bnd(G.nz, G.d, iz + 1);
+    const float* sb = sino + (int64_t)b * G.na * G.nu * nv;
+    double acc = 0.0;
+    for (int ia = 0; ia < G.na; ++ia) {
+        const SFFoot f = sf_foot<MODE>(G, cs[ia], sn[ia], ang[ia], ix, iy);
+        if (!(f.u1 > u0 - G.du) || !((f.u0 - u0) * inv_du < (float)G.nu + 1.0f)) continue;
+        int j0 = (int)fmaxf(floorf((f.u0 - u0) * inv_du) - 1.0f, 0.0f);
+        int j1 = (int)fminf(floorf((f.u1 - u0) * inv_du) + 1.0f, (float)(G.nu - 1));
+        int m0 = 0, m1 = 0;
+        float qa = 0.f, qb = 0.f;
+        if (is3) {
+            qa = f.cm * zl;
+            qb = f.cm * zr;
+            float lo = fminf(qa, qb), hi = fmaxf(qa, qb);
+            if (!(hi > v0 - G.dv) || !((lo - v0) * inv_dv < (float)nv + 1.0f)) continue;
+            m0 = (int)fmaxf(floorf((lo - v0) * inv_dv) - 1.0f, 0.0f);
+            m1 = (int)fminf(floorf((hi - v0) * inv_dv) + 1.0f, (float)(nv - 1));
+        }
+        const float* plane = sb + (int64_t)ia * G.nu * nv;
+        for (int j = j0; j <= j1; ++j) {
+            const float ou = ovl(f.u0, f.u1, sf_det(G.nu, G.du, G.su, j), sf_det(G.nu, G.du, G.su, j + 1));
+            if (!(ou > 0.0f)) continue;
+            const float wu = (ou * inv_du) * f.amp;
+            if (!is3) {
+                acc += (double)(plane[j] * wu);
+            } else {
+                for (int m = m0; m <= m1; ++m) {
+                    float ov = ovl(qa, qb, sf_det(nv, G.dv, G.sv, m), sf_det(nv, G.dv, G.sv, m + 1));
+                    if (ov > 0.0f) acc += (double)(plane[(int64_t)j * nv + m] * (wu * (ov * inv_dv)));
+                }
+            }
+        }
+    }
+    float res = (float)acc * G.out_scale;
+    if (accumulate) res += img[t];
+    img[t] = res;
+}
+
+static int sf_setup(const vir_sf_geom* g, SFGeo& G) {
+    if (!g) return fail_arg("geometry is NULL");
+    if (g->mode < 0 || g->mode > 3) return fail_arg("mode must be 0..3");
+    bool is3 = g->mode == SF_CONE3 || g->mode == SF_PAR3;
+    G.nx = g->nx; G.ny = g->ny; G.nz = is3 ? g->nz : 1;
+    G.nu = g->nu; G.nv = is3 ? g->nv : 1; G.na = g->na; G.nb = g->nbatch;
+    G.d = g->d_pix; G.du = g->du; G.dv = is3 ? g->dv : 1.0f;
+    G.su = g->su; G.sv = is3 ? g->sv : 0.0f; G.DSO = g->DSO; G.DSD = g->DSD;
+    if (G.nx < 1 || G.ny < 1 || G.nz < 1 || G.nu < 1 || G.nv < 1 || G.na < 0 || G.nb < 1)
+        return fail_arg("bad sizes");
+    if (!(G.d > 0.0f) || !(G.du > 0.0f) || !(G.dv > 0.0f)) return fail_arg("spacings must be > 0");
+    // Largest distance between a pixel's centre projection and any point of
+    // its footprint (the footprint contains the centre projection):
+    //   parallel: half the projected diagonal, d/sqrt(2);
+    //   fan/cone: a corner offset (dp, do) from the centre moves the
+    //   projection by DSD*(dp + t*do)/(DSO - o_corner) with t = u_c/DSD, so
+    //   the distance is at most mag * (d/sqrt(2)) * sqrt(1 + t^2), where
+    //   mag = DSD/(DSO - R) and |t| <= R/sqrt(DSO^2 - R^2) over the disc of
+    //   radius R (the grid's half-diagonal) that holds every pixel.
+    const double hx = 0.5 * G.nx * (double)G.d, hy = 0.5 * G.ny * (double)G.d;
+    const double R = std::sqrt(hx * hx + hy * hy);
+    double w = (double)G.d * 0.70710678118654752;
+    if (g->mode == SF_FAN2 || g->mode == SF_CONE3) {
+        if (!(G.DSO > 0.0f) || !(G.DSD > 0.0f)) return fail_arg("DSO and DSD must be > 0");
+        if (!(R < 0.999 * G.DSO)) return fail_arg("grid reaches the source (DSO too small)");
+        const double D = G.DSO;
+        w *= (double)G.DSD / (D - R) * (D / std::sqrt(D * D - R * R));
+    }
+    G.wmax = (float)(1.02 * w + 1e-3 * (double)G.du);
+    if (g->mode == SF_FAN2 || g->mode == SF_CONE3) {
+        // the forward gather inverts the centre projection along the driving
+        // axis; that inverse is monotone while |u| < DSD (fan half-angle
+        // below 45 degrees), so the detector plus margin must stay inside
+        const double umax = (double)G.du * std::fmax(std::fabs(-0.5 * G.nu + G.su),
+                                                     std::fabs(0.5 * G.nu + G.su)) + G.wmax;
+        if (!(umax < 0.999 * (double)G.DSD))
+            return fail_arg("detector reaches a 45 degree fan half-angle (|u| >= DSD)");
+    }
+    return VIR_OK;
+}
+
+}  // namespace vir
+
+using namespace vir;
+
+extern "C" {
+
+int vir_sf_fp(const vir_sf_geom* geom, const float* ang, const float* cosv, const float* sinv, const float* img,
+              float* sino, float out_scale, uint32_t flags, void* stream) {
+    SFGeo G;
+    int rc = sf_setup(geom, G);
+    if (rc) return rc;
+    if (G.na == 0) return VIR_OK;
+    if (!ang || !cosv || !sinv || !img || !sino) return fail_arg("NULL pointer argument");
+    G.out_scale = out_scale;
+    int64_t n = (int64_t)G.nb * G.na * G.nu * G.nv;
+    dim3 grd((unsigned)((n + 255) / 256));
+    hipStream_t st = as_stream(stream);
+    int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
+    switch (geom->mode) {
+        case SF_PAR2: hipLaunchKernelGGL(k_sf_fp<SF_PAR2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
+        case SF_FAN2: hipLaunchKernelGGL(k_sf_fp<SF_FAN2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
+        case SF_CONE3: hipLaunchKernelGGL(k_sf_fp<SF_CONE3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
+        default: hipLaunchKernelGGL(k_sf_fp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
+    }
+    return check_hip(hipGetLastError(), "vir_sf_fp launch");
+}
+
+int vir_sf_bp(const vir_sf_geom* geom, const float* ang, const float* cosv, const float* sinv, const float* sino,
+              float* img, float out_scale, uint32_t flags, void* stream) {
+    SFGeo G;
+    int rc = sf_setup(geom, G);
+    if (rc) return rc;
+    if ((G.na > 0 && (!ang || !cosv || !sinv || !sino)) || !img) return fail_arg("NULL pointer argument");
+    G.out_scale = out_scale;
+    int64_t n = (int64_t)G.nb * G.nx * G.ny * G.nz;
+    dim3 grd((unsigned)((n + 255) / 256));
+    hipStream_t st = as_stream(stream);
+    int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
+    switch (geom->mode) {
+        case SF_PAR2: hipLaunchKernelGGL(k_sf_bp<SF_PAR2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
+        case SF_FAN2: hipLaunchKernelGGL(k_sf_bp<SF_FAN2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
+        case SF_CONE3: hipLaunchKernelGGL(k_sf_bp<SF_CONE3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
+        default: hipLaunchKernelGGL(k_sf_bp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
+    }
+    return check_hip(hipGetLastError(), "vir_sf_bp launch");
+}
+
+}  // extern "C"
