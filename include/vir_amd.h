@@ -192,6 +192,29 @@ int vir_sf_bp(const vir_sf_geom *geom, const float *ang, const float *cosv,
               const float *sinv, const float *sino, float *img, float out_scale,
               uint32_t flags, void *stream);
 
+/* ------------------------------------------------------ ray geometry */
+
+/* Ray endpoints of a circular orbit, generated on the device
+ * (proj_geom.py:61-148 geom_circular).  mode: 0 par, 1 fan, 2 cone.
+ * cosv/sinv: per-view cos/sin in double (device); dets_y (nu) and dets_z
+ * (nv, or NULL when there is no DetsZ; required for cone) in double.
+ * src/trg: float32 (nviews, nu[, nv], 3), the float32 cast that siddons()
+ * applies to geom_circular's double endpoints.  Each view's rotation is
+ * evaluated as numpy's (N,2)x(2,2) matmul through OpenBLAS dgemm:
+ * x' = fma(y, -s, x*c), y' = fma(y, c, x*s). */
+int vir_geom_circular(int32_t mode, int64_t nviews, const double *cosv,
+                      const double *sinv, int32_t nu, const double *dets_y,
+                      int32_t nv, const double *dets_z, double src_iso,
+                      double det_iso, float *src, float *trg, void *stream);
+
+/* Synthetic ellipse/ellipsoid phantom on an (nx, ny, nz) grid in
+ * FOV-normalised coordinates (the style of phantoms.py): voxel value =
+ * sum of rho over the ellipsoids containing its centre.  params: nell x 16
+ * floats (centre 3, semi-axes 3, row-major rotation 9, rho), device.  No
+ * reference counterpart: the synthetic input of benchmarks and serving. */
+int vir_phantom(int32_t nx, int32_t ny, int32_t nz, int32_t nell,
+                const float *params, float *out, void *stream);
+
 /* ------------------------------------------------------------- SIRT ops */
 
 /* r[i] = W[i] != 0 ? (b[i] - r[i]) / W[i] : r[i]   (sys_mat/sirt_dd.py:23-26,

@@ -1,0 +1,160 @@
+"""Serving on the GPU and the device ray generator it relies on.
+
+* circular_rays: the device-generated float32 endpoints equal
+  geom_circular's (cast to float32) bit for bit; circular_projections fits
+  the same matrices as a fit over every ray;
+* a Siddon projector on device rays equals one on host rays;
+* a mix of fan- and cone-beam requests served concurrently: each result is
+  bit-identical to the same request run alone, and the SIRT iterates are
+  within 1e-5 relative L2 of the float64 recurrence with the same system
+  matrix.
+"""
+import numpy as np
+import pytest
+
+from oracle import dd_ref, sirt_ref
+from oracle import siddon as osd
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("geom", ["par", "fan", "cone"])
+def test_circular_rays_match_geom_circular(geom):
+    from vir_amd.geometry import circular_projections, circular_rays
+    from vir_amd.proj_geom import geom_circular, view_projections
+    views = np.arange(90) * (2 * np.pi / 90) + 0.0123
+    dy = (np.arange(128) - 63.5) * 0.7
+    dz = None if geom == "par" else (np.arange(48) - 23.5) * 0.9
+    kw = {} if geom == "par" else dict(src_iso=80.0, det_iso=70.0)
+    s_d, t_d = circular_rays(dy, views, geom, DetsZ=dz, **kw)
+    s_h, t_h = geom_circular(dy, views, geom, DetsZ=dz, **kw)
+    s_h, t_h = s_h.astype(np.float32), t_h.astype(np.float32)
+    assert s_d.shape == s_h.shape and t_d.shape == t_h.shape
+    assert np.array_equal(s_d.cpu().numpy(), s_h)
+    assert np.array_equal(t_d.cpu().numpy(), t_h)
+    if geom == "fan":
+        # a stack of fan slices (source z = DetsZ) is no single-source view:
+        # fit the projections of one fan
+        dz = None
+        s_d, t_d = circular_rays(dy, views, geom, **kw)
+        s_h, t_h = geom_circular(dy, views, geom, **kw)
+        s_h, t_h = s_h.astype(np.float32), t_h.astype(np.float32)
+        assert np.array_equal(s_d.cpu().numpy(), s_h) and np.array_equal(t_d.cpu().numpy(), t_h)
+    nv = 1 if dz is None else dz.size
+    full = view_projections(s_h.reshape(90, 128, nv, 3), t_h.reshape(90, 128, nv, 3))
+    fit = circular_projections(dy, views, geom, DetsZ=dz, **kw)
+    # same predicted ray index for points of the volume
+    rng = np.random.default_rng(0)
+    X = np.concatenate([rng.uniform(-30, 30, (200, 3)), np.ones((200, 1))], 1)
+    for a in (0, 17, 89):
+        Pf, Pc = full[a].reshape(3, 4).astype(np.float64), fit[a].reshape(3, 4).astype(np.float64)
+        for r in ((0, 1) if dz is not None else (0,)):
+            uf = (X @ Pf[r]) / (X @ Pf[2])
+            uc = (X @ Pc[r]) / (X @ Pc[2])
+            assert np.abs(uf - uc).max() < 1e-3
+
+
+def test_siddon_projector_on_device_rays():
+    import torch
+    from vir_amd.geometry import circular_projections, circular_rays
+    from vir_amd.proj_geom import geom_circular
+    from vir_amd.sys_mat.siddon import SiddonProjector
+    views = np.arange(24) * (2 * np.pi / 24)
+    dy = (np.arange(36) - 17.5) * 2.0
+    dz = (np.arange(30) - 14.5) * 2.0
+    kw = dict(src_iso=80.0, det_iso=80.0)
+    s_d, t_d = circular_rays(dy, views, "cone", DetsZ=dz, **kw)
+    mats = circular_projections(dy, views, "cone", DetsZ=dz, **kw)
+    Pd = SiddonProjector(s_d, t_d, (20, 20, 20), 1.0, proj=mats)
+    s_h, t_h = geom_circular(dy, views, "cone", DetsZ=dz, **kw)
+    Ph = SiddonProjector(s_h, t_h, (20, 20, 20), 1.0)
+    rng = np.random.default_rng(1)
+    x = torch.from_numpy(rng.uniform(0, 1, (20, 20, 20)).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.normal(size=(24, 36, 30)).astype(np.float32)).cuda()
+    assert torch.equal(Pd.fp(x), Ph.fp(x))
+    assert torch.equal(Pd.bp(y), Ph.bp(y))
+
+
+def _mix():
+    from vir_amd.serve import Request
+    reqs = []
+    spec = [("fan", 32, 30), ("cone", 16, 14), ("fan", 48, 40), ("fan", 32, 24),
+            ("cone", 24, 18), ("fan", 40, 36), ("cone", 16, 20), ("fan", 32, 30)]
+    for i, (k, n, v) in enumerate(spec):
+        reqs.append(Request(i, k, n, v, iters=12, arrival=0.002 * i, seed=100 + i))
+    return reqs
+
+
+def test_served_mix_equals_alone_and_fp64():
+    import torch
+    from vir_amd.serve import (Server, SirtBackend, cone_geometry, fan_geometry, run_alone,
+                               summarize)
+    from vir_amd.proj_geom import geom_circular
+    from vir_amd.sys_mat.dd import DDProjector
+    from vir_amd.sys_mat.siddon import SiddonProjector
+    from vir_amd.workloads import phantom_device
+    reqs = _mix()
+    be = SirtBackend(n_streams=4)
+    srv = Server(be, hbm_budget=8 << 30, max_active=4)
+    res = srv.run(reqs)
+    assert all(r.ok for r in res), [r.error for r in res if not r.ok]
+    assert srv.max_concurrent > 1
+    s = summarize(res, srv.elapsed)
+    assert s["completed"] == len(reqs) and s["p99_s"] >= s["p50_s"] > 0
+    for r, q in zip(res, reqs):
+        alone = run_alone(q)
+        assert np.array_equal(r.x, alone), q
+    # float64 recurrence with the same system matrix, one request per kind
+    q = reqs[0]
+    g = fan_geometry(q.n, q.views)
+    A = dd_ref.overlap_brute_2d(q.n, q.n, g["ang"].astype(np.float64), g["nu"],
+                                float(np.float32(g["du"])), 0.0, 1.0, geom="arc",
+                                DSO=float(np.float32(g["DSO"])), DSD=float(np.float32(g["DSD"])))
+    P = DDProjector(g["img_shape"], g["ang"], g["nu"], "fan", DSO=g["DSO"], DSD=g["DSD"],
+                    du=g["du"], detector="arc")
+    b = P.fp(phantom_device(g["img_shape"], 10, q.seed)).cpu().numpy()
+    ref = sirt_ref.sirt(A, b, q.iters)
+    assert rel(res[0].x, ref) <= TOL
+    q = reqs[1]
+    g = cone_geometry(q.n, q.views)
+    s_h, t_h = geom_circular(g["dets_y"], g["views"], "cone", g["src_iso"], g["det_iso"],
+                             g["dets_z"])
+    ind, ln, off = osd.siddons(s_h, t_h, g["img_shape"], 1.0, 0.0, flat=True)
+    A = sirt_ref.csr_matrix(ind, ln, off, q.n ** 3)
+    Ph = SiddonProjector(s_h, t_h, g["img_shape"], 1.0)
+    b = Ph.fp(phantom_device(g["img_shape"], 20, q.seed)).cpu().numpy()
+    ref = sirt_ref.sirt(A, b, q.iters)
+    assert rel(res[1].x, ref) <= TOL
+    torch.cuda.synchronize()
+
+
+def test_estimate_bounds_device_memory():
+    import torch
+    from vir_amd.serve import Request, SirtBackend, _Job
+    be = SirtBackend(n_streams=1)
+    for req in (Request(0, "fan", 64, 90, iters=2), Request(1, "cone", 32, 40, iters=2)):
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        job = _Job(req, be.streams[0])
+        job.result()
+        used = torch.cuda.memory_allocated() - base
+        assert used <= be.estimate(req), (req, used, be.estimate(req))
+        job.release()
+        del job
+
+
+def test_request_over_budget_is_rejected_and_others_run():
+    from vir_amd.serve import Request, Server, SirtBackend
+    be = SirtBackend(n_streams=2)
+    small = Request(0, "fan", 32, 20, iters=3)
+    big = Request(1, "cone", 64, 120, iters=3, arrival=0.0)
+    budget = be.estimate(small) * 2
+    res = Server(be, hbm_budget=budget, max_active=2).run([small, big])
+    assert res[0].ok and not res[1].ok and "budget" in res[1].error

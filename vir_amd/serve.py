@@ -1,0 +1,360 @@
+"""Serving SIRT reconstruction requests on one GPU.
+
+A request names a geometry (2-D fan-beam or 3-D cone-beam), a size, a view
+count, an iteration count and the seed of its synthetic object.  The server
+
+  * admits requests first-come first-served while the HBM budget and the
+    slot limit allow (a request larger than the whole budget is rejected;
+    nothing ever waits on a request behind it that can never fit);
+  * sets an admitted request up on a stream of its own: geometry on the
+    device, the measured sinogram (synthesised on the device: phantom, then
+    one forward projection), the normalisation sums W_sino = A 1 and
+    W_img = A^T 1, and one SIRT iteration captured into a HIP graph;
+  * enqueues all of its iterations as graph replays, then an asynchronous
+    copy of the result into pinned host memory, and an event;
+  * keeps polling: a request is complete when its event has fired; its
+    memory returns to the budget and waiting requests are admitted.
+
+Work of different requests overlaps on the device (one stream per slot);
+requests that arrive while others run are admitted as soon as memory and a
+slot free up (continuous batching at request granularity).  Every kernel on
+the path is deterministic, so a request's result does not depend on what
+else is running.
+
+Fan-beam requests use the distance-driven projector with an equiangular
+detector (sys_mat/dd.py, the reference's fan geometry on an arc); cone-beam
+requests use the on-the-fly Siddon projector with the brick-gather
+backprojector (sys_mat/siddon.py) on rays generated on the device
+(proj_geom.py geom_circular).  The recurrence is the reference's SIRT
+(sys_mat/sirt_dd.py:37-76, tests/Test_SIRT/sirt_test.py:21-40) from x0 = 0.
+
+The scheduling core (Server) is independent of the GPU: it drives a backend
+with estimate/start/poll/release, so it runs against a simulated backend
+on the CPU as well.
+"""
+import collections
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+FAN_NU_RATIO = 888.0 / 512.0
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str              # "fan" or "cone"
+    n: int                 # image n x n (fan) or volume n^3 (cone)
+    views: int
+    iters: int = 20
+    arrival: float = 0.0   # seconds after the start of the trace
+    seed: int = 0
+    gamma: float = 1.0
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    ok: bool
+    latency: float = math.nan      # arrival -> result on the host (s)
+    queued: float = math.nan       # arrival -> admission (s)
+    x: object = None               # host float32 array
+    error: str = ""
+
+
+# --------------------------------------------------------------- geometry
+
+def fan_geometry(n, views):
+    """2-D fan beam for an n x n image of unit pixels: the 512 / 720-view /
+    888-bin benchmark scaled with n (DSO = 1000 n/512, DSD = 1500 n/512),
+    round(1.734 n) equiangular bins covering the image's circumscribed
+    circle, views over [0, 2 pi)."""
+    DSO, DSD = 1000.0 * n / 512.0, 1500.0 * n / 512.0
+    nu = int(round(FAN_NU_RATIO * n))
+    gmax = math.asin((n / math.sqrt(2.0)) / DSO)
+    du = 2.0 * DSD * gmax * 1.02 / nu
+    ang = (np.arange(views) * (2 * np.pi / views)).astype(np.float32)
+    return dict(img_shape=(n, n), ang=ang, nu=nu, DSO=DSO, DSD=DSD, du=du)
+
+
+def cone_geometry(n, views):
+    """3-D cone beam for an n^3 volume of unit voxels: source-isocentre 4n,
+    source-detector 8n (magnification 2), round(1.5 n)^2 detector of pitch 2,
+    views over [0, 2 pi)."""
+    m = int(round(1.5 * n))
+    pitch = 2.0
+    dets = (np.arange(m) - (m - 1) / 2.0) * pitch
+    views_a = np.arange(views) * (2 * np.pi / views)
+    return dict(img_shape=(n, n, n), views=views_a, dets_y=dets, dets_z=dets,
+                src_iso=4.0 * n, det_iso=4.0 * n, m=m)
+
+
+# ------------------------------------------------------------ the server
+
+class Server:
+    """Admission and completion loop over a backend.
+
+    backend.estimate(req) -> bytes; backend.start(req, slot) -> handle;
+    handle.poll() -> bool (complete); handle.result() -> host array;
+    handle.release().  Slots index the backend's streams."""
+
+    def __init__(self, backend, hbm_budget, max_active=16, clock=time.perf_counter,
+                 idle_sleep=2e-4):
+        self.backend = backend
+        self.budget = int(hbm_budget)
+        self.max_active = int(max_active)
+        self.clock = clock
+        self.idle_sleep = idle_sleep
+        self.in_use = 0
+        self.max_in_use = 0
+        self.max_concurrent = 0
+
+    def run(self, requests, keep_results=True, progress=None):
+        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        waiting = collections.deque()
+        active = {}   # slot -> (req, handle, est, t_admit)
+        free_slots = list(range(self.max_active))[::-1]
+        results = {}
+        t0 = self.clock()
+        while pending or waiting or active:
+            now = self.clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            progressed = False
+            # completions first: they return memory and slots
+            for slot in list(active):
+                req, h, est, t_adm = active[slot]
+                if h.poll():
+                    t_done = self.clock() - t0
+                    x = h.result() if keep_results else None
+                    h.release()
+                    del active[slot]
+                    free_slots.append(slot)
+                    self.in_use -= est
+                    results[req.rid] = Result(req.rid, True, t_done - req.arrival,
+                                              t_adm - req.arrival, x)
+                    progressed = True
+                    if progress:
+                        progress(results[req.rid])
+            # FIFO admission
+            while waiting and free_slots:
+                req = waiting[0]
+                est = int(self.backend.estimate(req))
+                if est > self.budget:
+                    waiting.popleft()
+                    results[req.rid] = Result(req.rid, False, error=(
+                        f"request needs {est} bytes, more than the HBM budget {self.budget}"))
+                    progressed = True
+                    continue
+                if self.in_use + est > self.budget:
+                    break
+                waiting.popleft()
+                slot = free_slots.pop()
+                t_adm = self.clock() - t0
+                try:
+                    h = self.backend.start(req, slot)
+                except Exception as e:  # one failing request does not stop the others
+                    free_slots.append(slot)
+                    results[req.rid] = Result(req.rid, False, error=f"{type(e).__name__}: {e}")
+                    progressed = True
+                    continue
+                active[slot] = (req, h, est, t_adm)
+                self.in_use += est
+                self.max_in_use = max(self.max_in_use, self.in_use)
+                self.max_concurrent = max(self.max_concurrent, len(active))
+                progressed = True
+            if not progressed:
+                if not active and not waiting and pending:
+                    dt = pending[0].arrival - (self.clock() - t0)
+                    if dt > 0:
+                        time.sleep(min(dt, 0.05))
+                else:
+                    time.sleep(self.idle_sleep)
+        self.elapsed = self.clock() - t0
+        return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+
+def summarize(results, elapsed):
+    ok = [r for r in results if r.ok]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
+    return dict(jobs=len(results), completed=len(ok), rejected=len(results) - len(ok),
+                jobs_per_s=len(ok) / elapsed if elapsed > 0 else 0.0,
+                p50_s=float(np.percentile(lat, 50)) if ok else math.nan,
+                p99_s=float(np.percentile(lat, 99)) if ok else math.nan,
+                elapsed_s=elapsed)
+
+
+def poisson_trace(n_requests, rate, seed=0, iters=20, fan_frac=0.6,
+                  fan_sizes=(128, 256, 512, 1024), fan_views=(180, 720),
+                  cone_sizes=(64, 128, 256), cone_views=(120, 360)):
+    """Requests with exponential inter-arrival times (Poisson arrivals at
+    `rate` per second): a fan_frac share of 2-D fan-beam jobs (size uniform
+    over fan_sizes, views uniform in fan_views), the rest 3-D cone-beam."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    out = []
+    for i in range(n_requests):
+        if rng.uniform() < fan_frac:
+            kind, n = "fan", int(rng.choice(fan_sizes))
+            v = int(rng.integers(fan_views[0], fan_views[1] + 1))
+        else:
+            kind, n = "cone", int(rng.choice(cone_sizes))
+            v = int(rng.integers(cone_views[0], cone_views[1] + 1))
+        out.append(Request(i, kind, n, v, iters, float(t[i]), seed=int(rng.integers(1 << 30))))
+    return out
+
+
+# ------------------------------------------------------- the GPU backend
+
+class _Job:
+    def __init__(self, req, stream, graph=True):
+        import torch
+        from .geometry import circular_projections, circular_rays
+        from .sirt import SIRT
+        from .sys_mat.dd import DDProjector
+        from .sys_mat.siddon import SiddonProjector
+        from .workloads import phantom_device
+        self.req = req
+        self.stream = stream
+        with torch.cuda.stream(stream):
+            if req.kind == "fan":
+                g = fan_geometry(req.n, req.views)
+                P = DDProjector(g["img_shape"], g["ang"], g["nu"], "fan", DSO=g["DSO"],
+                                DSD=g["DSD"], du=g["du"], detector="arc")
+                n_ell = 10
+            elif req.kind == "cone":
+                g = cone_geometry(req.n, req.views)
+                src, trg = circular_rays(g["dets_y"], g["views"], "cone", g["src_iso"],
+                                         g["det_iso"], g["dets_z"])
+                mats = circular_projections(g["dets_y"], g["views"], "cone", g["src_iso"],
+                                            g["det_iso"], g["dets_z"])
+                P = SiddonProjector(src, trg, g["img_shape"], 1.0, proj=mats)
+                n_ell = 20
+            else:
+                raise ValueError(f"unknown request kind {req.kind!r}")
+            truth = phantom_device(g["img_shape"], n_ell, req.seed, stream=stream)
+            b = P.fp(truth)            # the measurement, synthesised on the device
+            del truth
+            S = SIRT(P, b, gamma=req.gamma)
+            self.P, self.S = P, S
+            self.host = torch.empty(S.img_shape, dtype=torch.float32, pin_memory=True)
+            if graph:
+                gr = torch.cuda.CUDAGraph()
+                gr.capture_begin()
+                S._iteration()
+                gr.capture_end()
+                self.graph = gr
+                for _ in range(req.iters):
+                    gr.replay()
+            else:
+                self.graph = None
+                for _ in range(req.iters):
+                    S._iteration()
+            self.host.copy_(S.x, non_blocking=True)
+            self.done = torch.cuda.Event()
+            self.done.record(stream)
+
+    def poll(self):
+        return self.done.query()
+
+    def result(self):
+        self.done.synchronize()
+        return self.host.numpy().copy()
+
+    def release(self):
+        self.graph = None
+        self.P = self.S = None
+
+
+class SirtBackend:
+    """Runs requests as device-resident, graph-captured SIRT jobs."""
+
+    def __init__(self, n_streams=8, graph=True):
+        import torch
+        from .device import require_gpu
+        require_gpu()
+        self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
+        self.graph = graph
+
+    @staticmethod
+    def estimate(req):
+        """Upper bound of the device bytes one request holds."""
+        from . import _lib
+        if req.kind == "fan":
+            g = fan_geometry(req.n, req.views)
+            nsino = req.views * g["nu"]
+            nvox = req.n * req.n
+            b = 4 * (4 * nsino + 4 * nvox) + 16 * req.views
+        else:
+            g = cone_geometry(req.n, req.views)
+            nrays = req.views * g["m"] * g["m"]
+            nvox = req.n ** 3
+            grid = _lib.make_grid((req.n,) * 3, (1.0,) * 3, (0.0,) * 3, 1e-5)
+            ws = int(_lib.lib().vir_siddon_bp_workspace(grid, req.views))
+            b = 24 * nrays + 4 * 4 * nrays + 4 * 4 * nvox + max(ws, 4 * nvox) + 4 * nvox
+            b += 48 * req.views + 64
+        # allocator granularity: every tensor rounded up to 2 MiB at most
+        return int(b * 1.02) + 24 * (2 << 20)
+
+    def start(self, req, slot):
+        return _Job(req, self.streams[slot % len(self.streams)], graph=self.graph)
+
+
+def run_alone(req, graph=True):
+    """One request on an idle device, synchronously: the reference result
+    for the alone-vs-in-the-mix check."""
+    import torch
+    job = _Job(req, torch.cuda.Stream(), graph=graph)
+    x = job.result()
+    job.release()
+    return x
+
+
+# ---------------------------------------------------- the per-call path
+
+def plain_sirt(req):
+    """The kernel-per-call path: the same projector kernels, but every
+    projection takes a host array and returns one (upload, kernel, download,
+    synchronise) and the SIRT arithmetic runs in numpy on the host, as the
+    reference's loops drive their projectors (sirt_test.py:25-35)."""
+    import torch
+    from .geometry import circular_projections, circular_rays
+    from .sys_mat.dd import DDProjector
+    from .sys_mat.siddon import SiddonProjector
+    from .workloads import phantom_device
+    if req.kind == "fan":
+        g = fan_geometry(req.n, req.views)
+        P = DDProjector(g["img_shape"], g["ang"], g["nu"], "fan", DSO=g["DSO"], DSD=g["DSD"],
+                        du=g["du"], detector="arc")
+        n_ell = 10
+    else:
+        g = cone_geometry(req.n, req.views)
+        src, trg = circular_rays(g["dets_y"], g["views"], "cone", g["src_iso"], g["det_iso"],
+                                 g["dets_z"])
+        mats = circular_projections(g["dets_y"], g["views"], "cone", g["src_iso"], g["det_iso"],
+                                    g["dets_z"])
+        P = SiddonProjector(src, trg, g["img_shape"], 1.0, proj=mats)
+        n_ell = 20
+
+    def fp(x):
+        return P.fp(torch.from_numpy(x).cuda()).cpu().numpy()
+
+    def bp(y):
+        return P.bp(torch.from_numpy(y).cuda()).cpu().numpy()
+
+    b = phantom_device(g["img_shape"], n_ell, req.seed)
+    b = P.fp(b).cpu().numpy()
+    shape = g["img_shape"]
+    W_sino = fp(np.ones(shape, np.float32))
+    W_img = bp(np.ones(b.shape, np.float32))
+    m1, m2 = W_sino != 0, W_img != 0
+    x = np.zeros(shape, np.float32)
+    for _ in range(req.iters):
+        r = fp(x)
+        r[m1] = (b[m1] - r[m1]) / W_sino[m1]
+        u = bp(r)
+        u[m2] /= W_img[m2]
+        x += np.float32(req.gamma) * u
+    return x

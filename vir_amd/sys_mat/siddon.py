@@ -129,40 +129,65 @@ class SiddonProjector:
     """
 
     def __init__(self, src, trg, nPixels=128, dPixels=1.0, origin=0.0, epsilon=1e-5,
-                 structured=None):
+                 structured=None, proj=None):
+        """proj: optional per-view projection matrices (nviews, 12) for rays
+        already on the device (float32 CUDA tensors of shape (nviews, nu[, nv],
+        3)), e.g. from vir_amd.geometry.circular_rays; the rays then stay
+        on the device."""
         require_gpu()
         n, d, o = grid_params(nPixels, dPixels, origin)
         self.nPixels = tuple(int(v) for v in n)
         self.grid = _lib.make_grid(n, d, o, epsilon)
-        src, trg, shape = _prep_rays(src, trg)
-        self.ray_shape = shape
-        self.nrays = int(np.prod(shape))
-        self.src = as_device_f32(src.reshape(-1, 3))
-        self.trg = as_device_f32(trg.reshape(-1, 3))
         self.nvox = int(np.prod(self.nPixels))
-        self._err = torch.zeros(1, dtype=torch.int32, device=self.src.device)
         self._csc = None
         self.proj = None
-        if structured is not False and len(shape) in (2, 3) and self.nrays > 0:
-            nviews, nu = shape[0], shape[1]
-            nv = shape[2] if len(shape) == 3 else 1
-            try:
-                mats = view_projections(src.reshape(nviews, nu, nv, 3),
-                                        trg.reshape(nviews, nu, nv, 3))
-                if not check_in_front(mats, n, d, o):
-                    raise ValueError("a view's source lies inside the volume's guard region")
-                self.proj = torch.from_numpy(mats).to(self.src.device)
-                self.layout = (nviews, nu, nv)
-            except ValueError:
-                if structured:
-                    raise
-                self.proj = None
+        if proj is not None:
+            if not (isinstance(src, torch.Tensor) and isinstance(trg, torch.Tensor)):
+                raise ValueError("proj= needs src and trg as device tensors")
+            shape = tuple(trg.shape[:-1])
+            if len(shape) not in (2, 3) or tuple(src.shape) != tuple(trg.shape):
+                raise ValueError("device rays must be (nviews, nu[, nv], 3)")
+            self.ray_shape = shape
+            self.nrays = int(np.prod(shape))
+            self.src = as_device_f32(src).reshape(-1, 3)
+            self.trg = as_device_f32(trg).reshape(-1, 3)
+            mats = np.asarray(proj, np.float32).reshape(shape[0], 12)
+            if not check_in_front(mats, n, d, o):
+                raise ValueError("a view's source lies inside the volume's guard region")
+            self.proj = torch.from_numpy(mats).to(self.src.device)
+            self.layout = (shape[0], shape[1], shape[2] if len(shape) == 3 else 1)
+        else:
+            src, trg, shape = _prep_rays(src, trg)
+            self.ray_shape = shape
+            self.nrays = int(np.prod(shape))
+            self.src = as_device_f32(src.reshape(-1, 3))
+            self.trg = as_device_f32(trg.reshape(-1, 3))
+            if structured is not False and len(shape) in (2, 3) and self.nrays > 0:
+                nviews, nu = shape[0], shape[1]
+                nv = shape[2] if len(shape) == 3 else 1
+                try:
+                    mats = view_projections(src.reshape(nviews, nu, nv, 3),
+                                            trg.reshape(nviews, nu, nv, 3))
+                    if not check_in_front(mats, n, d, o):
+                        raise ValueError("a view's source lies inside the volume's guard region")
+                    self.proj = torch.from_numpy(mats).to(self.src.device)
+                    self.layout = (nviews, nu, nv)
+                except ValueError:
+                    if structured:
+                        raise
+                    self.proj = None
+        self._err = torch.zeros(1, dtype=torch.int32, device=self.src.device)
         ws = 0
         if self.proj is not None:
             ws = int(_lib.lib().vir_siddon_bp_workspace(self.grid, self.layout[0]))
         # one extra volume lets VIR_ACCUMULATE stage through the workspace
         self._ws = torch.empty(max(ws, self.nvox * 4 if self.proj is not None else 0) // 4 + 1,
                                dtype=torch.float32, device=self.src.device)
+
+    def device_bytes(self):
+        """Device memory held by this projector (rays, matrices, workspace)."""
+        b = (self.src.numel() + self.trg.numel() + self._ws.numel()) * 4
+        return b + (self.proj.numel() * 4 if self.proj is not None else 0)
 
     # -- forward ---------------------------------------------------------
     def fp(self, vol, out=None, accumulate=False, check=False):

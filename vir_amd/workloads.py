@@ -66,6 +66,37 @@ def phantom(shape, n_ell, seed, supersample=1, device="cuda"):
     return out.to(torch.float32).reshape(shape).contiguous()
 
 
+def phantom_device(shape, n_ell, seed, device="cuda", stream=None):
+    """The phantom of `phantom` (same seeded ellipses, no supersampling)
+    evaluated by the vir_phantom kernel in float32: one launch, no
+    temporaries, enqueued on `stream` (default: the current stream)."""
+    from . import _lib
+    shape = tuple(int(s) for s in shape)
+    flat2d = len(shape) == 2 or (len(shape) == 3 and shape[2] == 1)
+    ndim = 2 if flat2d else 3
+    cen, ax, rot, rho = ellipse_params(n_ell, seed, ndim)
+    prm = np.zeros((n_ell, 16), np.float32)
+    prm[:, :ndim] = cen
+    prm[:, 3:3 + ndim] = ax
+    if ndim == 2:
+        prm[:, 5] = 1.0
+    R = np.zeros((n_ell, 3, 3))
+    R[:, :ndim, :ndim] = rot
+    if ndim == 2:
+        R[:, 2, 2] = 1.0
+    prm[:, 6:15] = R.reshape(n_ell, 9)
+    prm[:, 15] = rho
+    p = torch.from_numpy(prm).to(device, non_blocking=False)
+    nx, ny = shape[0], shape[1]
+    nz = 1 if ndim == 2 else shape[2]
+    out = torch.empty(shape, dtype=torch.float32, device=device)
+    st = _lib.stream_ptr(stream)
+    _lib.check(_lib.lib().vir_phantom(nx, ny, nz, n_ell, _lib.ptr(p), _lib.ptr(out), st), "vir_phantom")
+    if stream is not None:
+        p.record_stream(stream)   # keep the parameters alive until the kernel ran
+    return out
+
+
 def cone256_geometry(views=None):
     """3-D cone beam: 360 views over 2*pi, source-isocentre 500 mm,
     source-detector 1000 mm, 512 x 384 detector at 1.0 mm, 256^3 volume of
