@@ -34,6 +34,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the fan-beam SIRT, differentiable-projector and serving sections")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the Poisson trace replayed by the serving section")
+    ap.add_argument("--plain-requests", type=int, default=6,
+                    help="requests run through the kernel-per-call path for comparison")
     return ap.parse_args()
 
 
@@ -85,6 +91,149 @@ def cpu_baseline(src, trg, grid, target_s):
     return {"value": n / dt, "unit": "rays/s", "cores": 1, "kind": "port",
             "sample": f"{n} consecutive rays of the same scan (view {start // (512 * 384)}), "
                       f"C oracle siddons() trace + float64 CSR FP + BP, {dt:.1f} s"}
+
+
+def events(n):
+    import torch
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+def sirt_fan_section():
+    """512^2 ellipse phantom, 720 views over 2*pi, 888 equiangular bins
+    (DSO 1000, DSD 1500 pixel units), Poisson noise I0 = 1e5 on the
+    log-transformed sinogram, 50 graph-captured SIRT iterations with the
+    distance-driven fan projector."""
+    import torch
+    from vir_amd.serve import fan_geometry
+    from vir_amd.sirt import SIRT
+    from vir_amd.sys_mat.dd import DDProjector
+    from vir_amd.workloads import phantom
+    g = fan_geometry(512, 720)
+    P = DDProjector(g["img_shape"], g["ang"], g["nu"], "fan", DSO=g["DSO"], DSD=g["DSD"],
+                    du=g["du"], detector="arc")
+    x_true = phantom((512, 512), 10, seed=1003, supersample=4)
+    p = P.fp(x_true)
+    mu = 4.0 / float(p.max())
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1003)
+    I = torch.poisson(1e5 * torch.exp(-mu * p), generator=gen).clamp_(min=1.0)
+    b = -torch.log(I / 1e5) / mu
+    S = SIRT(P, b, graph=True)
+    S.step()                       # capture + first iteration
+    torch.cuda.synchronize()
+    S = SIRT(P, b, graph=True)
+    S.step()
+    torch.cuda.synchronize()
+    e0, e1 = events(2)
+    e0.record()
+    for _ in range(49):
+        S.step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_it = e0.elapsed_time(e1) / 49
+    n_views, n_d, n_v = 720, g["nu"], 512 * 512
+    b_fp = n_views * (4.0 * n_v + 4.0 * n_d)
+    alg = 2 * b_fp + 16.0 * n_d * n_views + 16.0 * n_v
+    rmse = float((S.x - x_true).norm() / x_true.norm())
+    return {"workload": "fan2d_dd_sirt_512", "views": n_views, "bins": n_d, "iterations": 50,
+            "ms_per_iteration": ms_it, "iterations_per_s": 1e3 / ms_it,
+            "ms_50_iterations": 50 * ms_it,
+            "alg_bytes_per_iteration": alg, "alg_GBps": alg / (ms_it * 1e-3) / 1e9,
+            "rel_error_vs_phantom": rmse, "noise": "Poisson I0=1e5, mu=4/max(p)"}
+
+
+def autograd_section():
+    """Batch of 8 random-ellipsoid volumes 256x256x64, 3-D parallel beam,
+    180 views over [0, pi), 384 x 64 detector at 1.0: 100 gradient steps on
+    0.5*||A x - b||^2 + 1e-3*TV(x), A the separable-footprint projector
+    inside torch.autograd."""
+    import torch
+    from vir_amd.autograd import least_squares, project, total_variation
+    from vir_amd.sys_mat.pd import SFProjector
+    from vir_amd.workloads import phantom_device
+    nb, shape = 8, (256, 256, 64)
+    ang = np.arange(180) * np.pi / 180
+    P = SFProjector(shape, ang, 384, "par3d", nv=64, du=1.0, dv=1.0, batch=nb)
+    truth = torch.stack([phantom_device(shape, 20, 1004 + k) for k in range(nb)])
+    b = P.fp(truth)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1004)
+    xr = torch.randn(truth.shape, device="cuda", generator=gen)
+    yr = torch.randn(b.shape, device="cuda", generator=gen)
+    lhs = float((P.fp(xr).double() * yr.double()).sum())
+    rhs = float((xr.double() * P.bp(yr).double()).sum())
+    adj = abs(lhs - rhs) / abs(lhs)
+    v = xr / xr.norm()
+    for _ in range(8):             # power iteration for ||A||^2
+        w = P.bp(P.fp(v))
+        L = float(w.norm())
+        v = w / L
+    lam, eps = 1e-3, 1e-2
+    lr = 1.0 / (L + lam * 12.0 / eps)
+
+    def loss_fn(x):
+        return least_squares(project(x, P), b) + lam * total_variation(x, eps=eps)
+
+    x = torch.zeros_like(truth)
+    l0 = float(loss_fn(x))
+    e0, e1 = events(2)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(100):
+        x.requires_grad_(True)
+        loss = loss_fn(x)
+        (gx,) = torch.autograd.grad(loss, x)
+        x = (x.detach() - lr * gx)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 100
+    l1 = float(loss_fn(x))
+    b_fp = nb * 180 * (4.0 * (256 * 256 * 64) + 4.0 * (384 * 64))
+    return {"workload": "par3d_sf_autograd_8x256x256x64", "steps": 100, "ms_per_step": ms,
+            "steps_per_s": 1e3 / ms, "adjoint_rel_err": adj, "loss_ratio_after_100": l1 / l0,
+            "alg_bytes_fp": b_fp, "lr": lr}
+
+
+def serving_section(n_serve, n_plain):
+    """The serving trace (Poisson arrivals at 200/s, 60 % fan-beam jobs of
+    128..1024^2 and 180..720 views, 40 % cone-beam jobs of 64..256^3 and
+    120..360 views, 20 SIRT iterations each): the first n_serve requests
+    replayed in real time under a 200 GB budget; the first n_plain
+    requests also run through the kernel-per-call path and through the
+    server with all arrivals at once."""
+    import dataclasses
+    from vir_amd.serve import Server, SirtBackend, plain_sirt, poisson_trace, summarize
+    trace = poisson_trace(10000, 200.0, seed=1005)
+    be = SirtBackend(n_streams=8)
+    warm = [dataclasses.replace(r, rid=i, arrival=0.0, iters=2) for i, r in
+            enumerate([q for q in trace[:40] if q.n <= 128][:4])]
+    Server(be, 200 << 30, max_active=16).run(warm, keep_results=False)
+    sample = trace[:n_serve]
+    srv = Server(be, 200 << 30, max_active=16)
+    res = srv.run(sample, keep_results=False,
+                  progress=lambda r: log(f"[serve] request {r.rid} done in {r.latency:.2f}s")
+                  if r.rid % 8 == 0 else None)
+    out = summarize(res, srv.elapsed)
+    out.update({"requests": n_serve, "arrival_rate_per_s": 200.0,
+                "max_concurrent": srv.max_concurrent, "max_in_use_GB": srv.max_in_use / 1e9,
+                "trace": "poisson_trace(10000, 200/s, seed 1005)[:%d]" % n_serve})
+    sub = [dataclasses.replace(r, arrival=0.0) for r in trace[:n_plain]]
+    t0 = time.perf_counter()
+    done = 0
+    for r in sub:
+        plain_sirt(r)
+        done += 1
+        log(f"[serve] plain request {r.rid} ({r.kind} {r.n} x {r.views}) done")
+        if time.perf_counter() - t0 > 90.0:
+            break
+    plain_s = time.perf_counter() - t0
+    srv2 = Server(be, 200 << 30, max_active=16)
+    res2 = srv2.run(sub[:done], keep_results=False)
+    served = summarize(res2, srv2.elapsed)
+    out["per_call_path"] = {"requests": done, "jobs_per_s": done / plain_s,
+                            "served_jobs_per_s_same_requests": served["jobs_per_s"],
+                            "served_over_per_call": served["jobs_per_s"] / (done / plain_s)}
+    return out
 
 
 def main():
@@ -188,6 +337,18 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(src, trg, grid, args.cpu_seconds)
+    if not args.no_extras:
+        del P, vol, sino, back, src, trg
+        torch.cuda.empty_cache()
+        log("[bench] fan-beam SIRT section")
+        out["sirt_fan"] = sirt_fan_section()
+        log(f"[bench] {out['sirt_fan']}")
+        log("[bench] differentiable projector section")
+        out["autograd"] = autograd_section()
+        log(f"[bench] {out['autograd']}")
+        torch.cuda.empty_cache()
+        log("[bench] serving section")
+        out["serving"] = serving_section(args.serve_requests, args.plain_requests)
     print(json.dumps(out), flush=True)
 
 

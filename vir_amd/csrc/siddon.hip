@@ -143,25 +143,25 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
 // footprint apart, so the lanes of a wave never update the same voxel in the
 // same colour phase, and consecutive phases of one wave are ordered by the
 // in-order LDS pipeline -- no atomics and no workgroup barriers.
-template <int SBX, int SBY, int SBZ, bool P2, bool ISO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_siddon_bp_wave(GridK g, const float* __restrict__ src,
+template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, bool P2, bool ISO>
+__global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WPE))) void k_siddon_bp_wave(GridK g, const float* __restrict__ src,
                                                       const float* __restrict__ trg, int nviews, int nu,
                                                       int nv, const float* __restrict__ proj,
                                                       const float* __restrict__ sino,
                                                       float* __restrict__ out, int views_per_chunk,
                                                       int nbx, int nby, int nbz, int64_t out_stride) {
     constexpr int kVox = SBX * SBY * SBZ;
-    __shared__ float acc_all[4 * kVox];
+    __shared__ float acc_all[WX * WY * kVox];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     float* acc = acc_all + wave * kVox;
 
-    // nbx/nby/nbz count workgroup regions (2*SBX x 2*SBY x SBZ)
+    // nbx/nby/nbz count workgroup regions (WX*SBX x WY*SBY x SBZ)
     const int region = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int rz = region % nbz;
     const int ry = (region / nbz) % nby;
     const int rx = region / (nbz * nby);
-    const int b0[3] = {rx * 2 * SBX + (wave & 1) * SBX, ry * 2 * SBY + (wave >> 1) * SBY, rz * SBZ};
+    const int b0[3] = {rx * WX * SBX + (wave % WX) * SBX, ry * WY * SBY + (wave / WX) * SBY, rz * SBZ};
     const int b1[3] = {min(b0[0] + SBX, g.n[0]), min(b0[1] + SBY, g.n[1]), min(b0[2] + SBZ, g.n[2])};
     const bool live = b0[0] < g.n[0] && b0[1] < g.n[1];
 
@@ -339,19 +339,35 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int6
 // 16x16x8 voxels (8 KiB of f32 accumulators per wave), flat (nZ == 1) grids
 // 16x16x1.  Views are split into chunks until there are enough workgroups to
 // fill the chip; chunk partial volumes are summed in a fixed order.
+// Sub-brick shapes of the backprojector: per-wave brick SBX x SBY x SBZ,
+// WX x WY waves per workgroup, WPE waves per SIMD the register budget allows.
+// 16 x 16 x 8 with 2 x 2 waves was the fastest of the shapes measured for
+// the 256^3 cone-beam case (8^3 to 32 x 32 x 8, one to four waves).
+template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE>
+struct BpShape {
+    static constexpr int sbx = SBX, sby = SBY, sbz = SBZ, wx = WX, wy = WY;
+};
+using BpShape3 = BpShape<16, 16, 8, 2, 2, 4>;
+using BpShape2 = BpShape<16, 16, 1, 2, 2, 4>;
+
 struct BpPlan {
     int kind;  // 0: 3D sub-bricks, 1: 2D sub-bricks
-    int sbx, sby, sbz;
-    int nbx, nby, nbz;  // workgroup regions (2*sbx x 2*sby x sbz)
+    int sbx, sby, sbz, wx, wy;
+    int nbx, nby, nbz;  // workgroup regions (wx*sbx x wy*sby x sbz)
     int nchunks, views_per_chunk;
 };
 
 static BpPlan plan_bp(const GridK& g, int nviews) {
     BpPlan p;
-    if (g.n[2] == 1) { p.kind = 1; p.sbx = 16; p.sby = 16; p.sbz = 1; }
-    else { p.kind = 0; p.sbx = 16; p.sby = 16; p.sbz = 8; }
-    p.nbx = (g.n[0] + 2 * p.sbx - 1) / (2 * p.sbx);
-    p.nby = (g.n[1] + 2 * p.sby - 1) / (2 * p.sby);
+    if (g.n[2] == 1) {
+        p.kind = 1; p.sbx = BpShape2::sbx; p.sby = BpShape2::sby; p.sbz = BpShape2::sbz;
+        p.wx = BpShape2::wx; p.wy = BpShape2::wy;
+    } else {
+        p.kind = 0; p.sbx = BpShape3::sbx; p.sby = BpShape3::sby; p.sbz = BpShape3::sbz;
+        p.wx = BpShape3::wx; p.wy = BpShape3::wy;
+    }
+    p.nbx = (g.n[0] + p.wx * p.sbx - 1) / (p.wx * p.sbx);
+    p.nby = (g.n[1] + p.wy * p.sby - 1) / (p.wy * p.sby);
     p.nbz = (g.n[2] + p.sbz - 1) / p.sbz;
     int64_t nb = (int64_t)p.nbx * p.nby * p.nbz;
     const int target = 1024;
@@ -467,17 +483,30 @@ int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu
     int nb = p.nbx * p.nby * p.nbz;
     dim3 grd((unsigned)nb, (unsigned)p.nchunks);
     const bool p2 = all_pow2(g), iso = isotropic(g);
-#define VIR_BP_LAUNCH(SX, SY, SZ)                                                                             \
-    do {                                                                                                        \
-        if (p2 && iso) hipLaunchKernelGGL((k_siddon_bp_wave<SX, SY, SZ, true, true>), grd, dim3(256), 0, st,      \
-                                          VIR_BP_ARGS);                                                         \
-        else if (p2) hipLaunchKernelGGL((k_siddon_bp_wave<SX, SY, SZ, true, false>), grd, dim3(256), 0, st,       \
-                                        VIR_BP_ARGS);                                                           \
-        else hipLaunchKernelGGL((k_siddon_bp_wave<SX, SY, SZ, false, false>), grd, dim3(256), 0, st, VIR_BP_ARGS); \
+#define VIR_BP_LAUNCH(SHAPE)                                                                                   \
+    do {                                                                                                         \
+        dim3 blk(64 * (SHAPE##_WX) * (SHAPE##_WY));                                                              \
+        if (p2 && iso) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, true>), grd, blk, 0, st,          \
+                                          VIR_BP_ARGS);                                                          \
+        else if (p2) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, false>), grd, blk, 0, st,           \
+                                        VIR_BP_ARGS);                                                            \
+        else hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, false, false>), grd, blk, 0, st, VIR_BP_ARGS);     \
     } while (0)
 #define VIR_BP_ARGS g, src, trg, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
-    if (p.kind == 0) VIR_BP_LAUNCH(16, 16, 8);
-    else VIR_BP_LAUNCH(16, 16, 1);
+#define BP3_ARGS 16, 16, 8, 2, 2, 4
+#define BP3_WX BpShape3::wx
+#define BP3_WY BpShape3::wy
+#define BP2_ARGS 16, 16, 1, 2, 2, 4
+#define BP2_WX 2
+#define BP2_WY 2
+    if (p.kind == 0) VIR_BP_LAUNCH(BP3);
+    else VIR_BP_LAUNCH(BP2);
+#undef BP3_ARGS
+#undef BP3_WX
+#undef BP3_WY
+#undef BP2_ARGS
+#undef BP2_WX
+#undef BP2_WY
 #undef VIR_BP_ARGS
 #undef VIR_BP_LAUNCH
     rc = check_hip(hipGetLastError(), "vir_siddon_bp launch");
