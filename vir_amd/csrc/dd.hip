@@ -79,13 +79,21 @@ __device__ __forceinline__ float dd_scale(const DDGeo& G, const DDView& V, float
     return sqrtf(den * den + po * po) / (den * V.rs);
 }
 
-// inverse of dd_bound along the driving axis: the drive coordinate whose
-// boundary projects to u (approximate; callers widen the index range)
+// detector coordinate u as the slope t = p / (DSO - o) of its ray (fan), or
+// u itself (parallel): the per-bin part of dd_invert, hoisted out of loops
 template <int MODE>
-__device__ __forceinline__ float dd_invert(const DDGeo& G, const DDView& V, float u, float orth_c) {
+__device__ __forceinline__ float dd_slope(const DDGeo& G, float u) {
+    if (MODE == DD_PAR) return u;
+    return MODE == DD_FAN_FLAT ? u / G.DSD : tanf(u / G.DSD);
+}
+
+// inverse of dd_bound along the driving axis: the drive coordinate whose
+// boundary projects to the detector coordinate of slope t (rounding only;
+// callers widen the index range by one)
+template <int MODE>
+__device__ __forceinline__ float dd_invert(const DDGeo& G, const DDView& V, float t, float orth_c) {
     float po = V.po * orth_c, oo = V.oo * orth_c;
-    if (MODE == DD_PAR) return (u - po) / V.pd;
-    float t = MODE == DD_FAN_FLAT ? u / G.DSD : tanf(u / G.DSD);
+    if (MODE == DD_PAR) return (t - po) / V.pd;
     return (t * (G.DSO - oo) - po) / (V.pd + t * V.od);
 }
 
@@ -107,15 +115,16 @@ __global__ __launch_bounds__(256) void k_dd2d_fp(DDGeo G, const float* __restric
     const DDView V = dd_view(cs[ia], sn[ia]);
     const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
     const float ul = det_bnd(G.nu, G.du, G.su, iu), ur = det_bnd(G.nu, G.du, G.su, iu + 1);
+    const float tl = dd_slope<MODE>(G, ul), tr = dd_slope<MODE>(G, ur);
     const float inv_d = 1.0f / G.d;
     double acc = 0.0;
     for (int io = 0; io < no; ++io) {
         const float oc = cell_cen(no, G.d, io);
-        float x0 = dd_invert<MODE>(G, V, ul, oc), x1 = dd_invert<MODE>(G, V, ur, oc);
+        float x0 = dd_invert<MODE>(G, V, tl, oc), x1 = dd_invert<MODE>(G, V, tr, oc);
         float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd;
         float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd;
-        int k0 = (int)fmaxf(floorf(k0f) - 2.0f, 0.0f);
-        int k1 = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
+        int k0 = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
+        int k1 = (int)fminf(floorf(k1f) + 1.0f, (float)(nd - 1));
         if (!(k0f < (float)nd + 2.0f) || !(k1f > -2.0f)) continue;
         const float sc = dd_scale<MODE>(G, V, oc);
         float pb = dd_bound<MODE>(G, V, cell_bnd(nd, G.d, k0), oc);
@@ -188,21 +197,22 @@ __global__ __launch_bounds__(256) void k_ddcone_fp(DDGeo G, const float* __restr
     const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
     const float ul = det_bnd(G.nu, G.du, G.su, iu), ur = det_bnd(G.nu, G.du, G.su, iu + 1);
     const float vl = det_bnd(G.nv, G.dv, G.sv, iv), vr = det_bnd(G.nv, G.dv, G.sv, iv + 1);
+    const float tl = dd_slope<DD_FAN_FLAT>(G, ul), tr = dd_slope<DD_FAN_FLAT>(G, ur);
     const float inv_d = 1.0f / G.d;
     double acc = 0.0;
     for (int io = 0; io < no; ++io) {
         const float oc = cell_cen(no, G.d, io);
-        float x0 = dd_invert<DD_FAN_FLAT>(G, V, ul, oc), x1 = dd_invert<DD_FAN_FLAT>(G, V, ur, oc);
+        float x0 = dd_invert<DD_FAN_FLAT>(G, V, tl, oc), x1 = dd_invert<DD_FAN_FLAT>(G, V, tr, oc);
         float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd;
         float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd;
         if (!(k0f < (float)nd + 2.0f) || !(k1f > -2.0f)) continue;
         const float zm = cone_zmag(G, V, oc);
         float z0f = vl / zm * inv_d + 0.5f * (float)G.nz, z1f = vr / zm * inv_d + 0.5f * (float)G.nz;
         if (!(z0f < (float)G.nz + 2.0f) || !(z1f > -2.0f)) continue;
-        int k0 = (int)fmaxf(floorf(k0f) - 2.0f, 0.0f);
-        int k1 = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
-        int z0 = (int)fmaxf(floorf(z0f) - 2.0f, 0.0f);
-        int z1 = (int)fminf(floorf(z1f) + 2.0f, (float)(G.nz - 1));
+        int k0 = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
+        int k1 = (int)fminf(floorf(k1f) + 1.0f, (float)(nd - 1));
+        int z0 = (int)fmaxf(floorf(z0f) - 1.0f, 0.0f);
+        int z1 = (int)fminf(floorf(z1f) + 1.0f, (float)(G.nz - 1));
         const float sc = dd_scale<DD_FAN_FLAT>(G, V, oc);
         float pb = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, k0), oc);
         for (int k = k0; k <= k1; ++k) {

@@ -43,21 +43,44 @@ struct SFFoot {
     float cm;       // v magnification (cone) or 1
 };
 
-// Footprint of pixel (ix, iy) in view (c, s, ang).
+// Per-view constants.
+struct SFView {
+    float c, s, ang;
+    float amp;  // 1/(|sin| + |cos|)
+    float h;    // half the projected width of a pixel in parallel beam
+};
+
+__device__ __forceinline__ SFView sf_view(const SFGeo& G, float c, float s, float ang) {
+    SFView V;
+    V.c = c;
+    V.s = s;
+    V.ang = ang;
+    const float w = fabsf(s) + fabsf(c);
+    V.amp = 1.0f / w;
+    V.h = 0.5f * G.d * w;
+    return V;
+}
+
+// Footprint of pixel (ix, iy) in view V.  In parallel beam the four
+// projected corners -s*x + c*y span exactly [p_c - h, p_c + h] around the
+// centre's projection p_c, h = d(|sin| + |cos|)/2.
 template <int MODE>
-__device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, float c, float s, float ang, int ix, int iy) {
+__device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, const SFView& V, int ix, int iy) {
+    const float c = V.c, s = V.s;
+    SFFoot f;
+    f.cm = 1.0f;
+    f.amp = V.amp;
+    if (MODE == SF_PAR2 || MODE == SF_PAR3) {
+        const float xc = G.d * (((float)ix + 0.5f) - 0.5f * (float)G.nx);
+        const float yc = G.d * (((float)iy + 0.5f) - 0.5f * (float)G.ny);
+        const float pc = -s * xc + c * yc;
+        f.u0 = pc - V.h;
+        f.u1 = pc + V.h;
+        return f;
+    }
     const float xl = sf_bnd(G.nx, G.d, ix), xr = sf_bnd(G.nx, G.d, ix + 1);
     const float yl = sf_bnd(G.ny, G.d, iy), yr = sf_bnd(G.ny, G.d, iy + 1);
     const float pxl = -s * xl, pxr = -s * xr, pyl = c * yl, pyr = c * yr;
-    SFFoot f;
-    f.cm = 1.0f;
-    if (MODE == SF_PAR2 || MODE == SF_PAR3) {
-        float p0 = pyl + pxl, p1 = pyl + pxr, p2 = pyr + pxl, p3 = pyr + pxr;
-        f.u0 = fminf(fminf(p0, p1), fminf(p2, p3));
-        f.u1 = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
-        f.amp = 1.0f / (fabsf(s) + fabsf(c));
-        return f;
-    }
     const float oxl = c * xl, oxr = c * xr, oyl = s * yl, oyr = s * yr;
     const float pxc = (pxl + pxr) * 0.5f, oxc = (oxl + oxr) * 0.5f;
     const float pyc = (pyl + pyr) * 0.5f, oyc = (oyl + oyr) * 0.5f;
@@ -71,7 +94,7 @@ __device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, float c, float s, floa
         float p3 = G.DSD * (pxr + pyr) / (G.DSO - (oxr + oyr));
         f.u0 = fminf(fminf(p0, p1), fminf(p2, p3));
         f.u1 = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
-        float rx = cosf(ang + gam), ry = sinf(ang + gam);
+        float rx = cosf(V.ang + gam), ry = sinf(V.ang + gam);
         f.amp = 1.0f / (fabsf(rx) + fabsf(ry));
         return f;
     }
@@ -81,9 +104,18 @@ __device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, float c, float s, floa
     f.u0 = fminf(fminf(u0, u1), fminf(u2, u3));
     f.u1 = fmaxf(fmaxf(u0, u1), fmaxf(u2, u3));
     f.cm = fm / cosf(gam);
-    f.amp = 1.0f / (fabsf(s) + fabsf(c));
     return f;
 }
+
+// Rows (FP: slabs) a slab (row) of the parallel 3-D geometry meets, with
+// their weights overlap/dv: independent of the pixel and the view, so
+// computed once per thread.  Up to kSlabs entries; more (rows much taller
+// than voxels) fall back to the general loop.
+constexpr int kSlabs = 4;
+struct SlabWeights {
+    int first, count;
+    float w[kSlabs];
+};
 
 __device__ __forceinline__ float ovl(float a0, float a1, float b0, float b1) {
     return fminf(a1, b1) - fmaxf(a0, b0);
@@ -116,7 +148,8 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
     const int iv = (int)(r % nv);
     const int iu = (int)((r / nv) % G.nu);
     const int ia = (int)(r / ((int64_t)nv * G.nu));
-    const float c = cs[ia], s = sn[ia], a = ang[ia];
+    const SFView V = sf_view(G, cs[ia], sn[ia], ang[ia]);
+    const float c = V.c, s = V.s;
     const bool xd = fabsf(s) >= fabsf(c);
     const int nd = xd ? G.nx : G.ny, no = xd ? G.ny : G.nx;
     const float ul = sf_det(G.nu, G.du, G.su, iu), ur = sf_det(G.nu, G.du, G.su, iu + 1);
@@ -124,6 +157,29 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
     const float inv_d = 1.0f / G.d, inv_du = 1.0f / G.du, inv_dv = 1.0f / G.dv;
     const int nz = is3 ? G.nz : 1;
     const float* vol = img + (int64_t)b * G.nx * G.ny * nz;
+    // parallel 3-D: the slabs of this detector row and their weights
+    SlabWeights SW;
+    SW.first = 0;
+    SW.count = 0;
+    bool slab_general = false;
+    if (MODE == SF_PAR3) {
+        float z0f = fminf(vl, vr) * inv_d + 0.5f * (float)G.nz;
+        float z1f = fmaxf(vl, vr) * inv_d + 0.5f * (float)G.nz;
+        int z0 = (int)fmaxf(floorf(z0f) - 1.0f, 0.0f);
+        int z1 = (int)fminf(floorf(z1f) + 1.0f, (float)(G.nz - 1));
+        SW.first = z0;
+        if (z1 - z0 + 1 > kSlabs) {
+            slab_general = true;
+        } else {
+#pragma unroll
+            for (int k = 0; k < kSlabs; ++k) {
+                const int z = z0 + k;
+                float ov = z <= z1 ? ovl(sf_bnd(G.nz, G.d, z), sf_bnd(G.nz, G.d, z + 1), vl, vr) : 0.0f;
+                SW.w[k] = ov > 0.0f ? ov * inv_dv : 0.0f;
+            }
+            SW.count = z1 - z0 + 1;
+        }
+    }
     double acc = 0.0;
     for (int io = 0; io < no; ++io) {
         const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
@@ -136,13 +192,19 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
         int k1 = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
         for (int k = k0; k <= k1; ++k) {
             const int ix = xd ? k : io, iy = xd ? io : k;
-            const SFFoot f = sf_foot<MODE>(G, c, s, a, ix, iy);
+            const SFFoot f = sf_foot<MODE>(G, V, ix, iy);
             const float ou = ovl(f.u0, f.u1, ul, ur);
             if (!(ou > 0.0f)) continue;
             const float wu = (ou * inv_du) * f.amp;
             const float* col = vol + ((int64_t)ix * G.ny + iy) * nz;
             if (!is3) {
                 acc += (double)(col[0] * wu);
+            } else if (MODE == SF_PAR3 && !slab_general) {
+                float sv_ = 0.0f;
+#pragma unroll
+                for (int q = 0; q < kSlabs; ++q)
+                    if (SW.w[q] > 0.0f) sv_ += col[SW.first + q] * SW.w[q];
+                acc += (double)(wu * sv_);
             } else {
                 // slabs whose magnified z interval meets [vl, vr]
                 float z0f = fminf(vl, vr) / f.cm * inv_d + 0.5f * (float)G.nz;
@@ -180,34 +242,70 @@ __global__ __launch_bounds__(256) void k_sf_bp(SFGeo G, const float* __restrict_
     const float inv_du = 1.0f / G.du, inv_dv = 1.0f / G.dv;
     const float zl = sf_bnd(G.nz, G.d, iz), zr = sf_bnd(G.nz, G.d, iz + 1);
     const float* sb = sino + (int64_t)b * G.na * G.nu * nv;
+    // parallel 3-D: the detector rows this slab meets and their weights
+    SlabWeights SW;
+    SW.first = 0;
+    SW.count = 0;
+    bool row_general = false;
+    if (MODE == SF_PAR3) {
+        const float lo = fminf(zl, zr), hi = fmaxf(zl, zr);
+        int m0 = (int)fmaxf(floorf((lo - v0) * inv_dv) - 1.0f, 0.0f);
+        int m1 = (int)fminf(floorf((hi - v0) * inv_dv) + 1.0f, (float)(nv - 1));
+        SW.first = m0;
+        if (m1 - m0 + 1 > kSlabs) {
+            row_general = true;
+        } else {
+#pragma unroll
+            for (int k = 0; k < kSlabs; ++k) {
+                const int m = m0 + k;
+                float ov = m <= m1 ? ovl(zl, zr, sf_det(nv, G.dv, G.sv, m), sf_det(nv, G.dv, G.sv, m + 1)) : 0.0f;
+                SW.w[k] = ov > 0.0f ? ov * inv_dv : 0.0f;
+            }
+            SW.count = m1 - m0 + 1;
+        }
+    }
     double acc = 0.0;
     for (int ia = 0; ia < G.na; ++ia) {
-        const SFFoot f = sf_foot<MODE>(G, cs[ia], sn[ia], ang[ia], ix, iy);
+        const SFView V = sf_view(G, cs[ia], sn[ia], ang[ia]);
+        const SFFoot f = sf_foot<MODE>(G, V, ix, iy);
         if (!(f.u1 > u0 - G.du) || !((f.u0 - u0) * inv_du < (float)G.nu + 1.0f)) continue;
         int j0 = (int)fmaxf(floorf((f.u0 - u0) * inv_du) - 1.0f, 0.0f);
         int j1 = (int)fminf(floorf((f.u1 - u0) * inv_du) + 1.0f, (float)(G.nu - 1));
-        int m0 = 0, m1 = 0;
-        float qa = 0.f, qb = 0.f;
-        if (is3) {
-            qa = f.cm * zl;
-            qb = f.cm * zr;
-            float lo = fminf(qa, qb), hi = fmaxf(qa, qb);
-            if (!(hi > v0 - G.dv) || !((lo - v0) * inv_dv < (float)nv + 1.0f)) continue;
-            m0 = (int)fmaxf(floorf((lo - v0) * inv_dv) - 1.0f, 0.0f);
-            m1 = (int)fminf(floorf((hi - v0) * inv_dv) + 1.0f, (float)(nv - 1));
-        }
         const float* plane = sb + (int64_t)ia * G.nu * nv;
+        if (!is3) {
+            for (int j = j0; j <= j1; ++j) {
+                const float ou = ovl(f.u0, f.u1, sf_det(G.nu, G.du, G.su, j), sf_det(G.nu, G.du, G.su, j + 1));
+                if (ou > 0.0f) acc += (double)(plane[j] * ((ou * inv_du) * f.amp));
+            }
+            continue;
+        }
+        if (MODE == SF_PAR3 && !row_general) {
+            if (SW.count <= 0) continue;
+            for (int j = j0; j <= j1; ++j) {
+                const float ou = ovl(f.u0, f.u1, sf_det(G.nu, G.du, G.su, j), sf_det(G.nu, G.du, G.su, j + 1));
+                if (!(ou > 0.0f)) continue;
+                const float wu = (ou * inv_du) * f.amp;
+                const float* rowp = plane + (int64_t)j * nv + SW.first;
+                float sv_ = 0.0f;
+#pragma unroll
+                for (int q = 0; q < kSlabs; ++q)
+                    if (SW.w[q] > 0.0f) sv_ += rowp[q] * SW.w[q];
+                acc += (double)(wu * sv_);
+            }
+            continue;
+        }
+        float qa = f.cm * zl, qb = f.cm * zr;
+        float lo = fminf(qa, qb), hi = fmaxf(qa, qb);
+        if (!(hi > v0 - G.dv) || !((lo - v0) * inv_dv < (float)nv + 1.0f)) continue;
+        int m0 = (int)fmaxf(floorf((lo - v0) * inv_dv) - 1.0f, 0.0f);
+        int m1 = (int)fminf(floorf((hi - v0) * inv_dv) + 1.0f, (float)(nv - 1));
         for (int j = j0; j <= j1; ++j) {
             const float ou = ovl(f.u0, f.u1, sf_det(G.nu, G.du, G.su, j), sf_det(G.nu, G.du, G.su, j + 1));
             if (!(ou > 0.0f)) continue;
             const float wu = (ou * inv_du) * f.amp;
-            if (!is3) {
-                acc += (double)(plane[j] * wu);
-            } else {
-                for (int m = m0; m <= m1; ++m) {
-                    float ov = ovl(qa, qb, sf_det(nv, G.dv, G.sv, m), sf_det(nv, G.dv, G.sv, m + 1));
-                    if (ov > 0.0f) acc += (double)(plane[(int64_t)j * nv + m] * (wu * (ov * inv_dv)));
-                }
+            for (int m = m0; m <= m1; ++m) {
+                float ov = ovl(qa, qb, sf_det(nv, G.dv, G.sv, m), sf_det(nv, G.dv, G.sv, m + 1));
+                if (ov > 0.0f) acc += (double)(plane[(int64_t)j * nv + m] * (wu * (ov * inv_dv)));
             }
         }
     }
