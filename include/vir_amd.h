@@ -192,6 +192,38 @@ int vir_sf_bp(const vir_sf_geom *geom, const float *ang, const float *cosv,
               const float *sinv, const float *sino, float *img, float out_scale,
               uint32_t flags, void *stream);
 
+/* ------------------------------------------ ray-driven (AW / Joseph) */
+
+/* Geometry of the ray-driven projectors (sys_mat/rd.py).  mode: 0 parallel
+ * 2-D (rd.py:472-541), 1 fan 2-D, flat detector (rd.py:566-649), 2 cone 3-D
+ * (rd.py:677-790).  joseph: 0 Amanatides-Woo path lengths, 1 Joseph
+ * interpolation (0.5-unit samples).  Images (nx, ny[, nz]) centred at the
+ * origin with pixel d_pix; sinograms (na, nu[, nv]); cosv/sinv per view
+ * (device). */
+typedef struct vir_rd_geom {
+    int32_t nx, ny, nz;
+    int32_t nu, nv, na;
+    int32_t mode;
+    int32_t joseph;
+    float d_pix, du, dv, su, sv, DSO, DSD;
+} vir_rd_geom;
+
+/* sino = out_scale * ray sums (AW: value * path length, cone scaled by
+ * DSD/sqrt(DSD^2+u^2+v^2); Joseph: 0.5 * interpolated samples); one thread
+ * per ray.  The reference's forward projectors use out_scale = 1. */
+int vir_rd_fp(const vir_rd_geom *geom, const float *cosv, const float *sinv,
+              const float *img, float *sino, float out_scale, uint32_t flags,
+              void *stream);
+
+/* The reference's ray-driven backprojection as a gather (one thread per
+ * voxel, no atomics): AW path-length splat (rd.py:151-175, 215-247) or
+ * Joseph bilinear/trilinear splat (rd.py:356-445).  The reference scales
+ * by du/(na*d_pix^2) (2-D, rd.py:538-541, 646-649) or du*dv/na (cone,
+ * rd.py:787-788). */
+int vir_rd_bp(const vir_rd_geom *geom, const float *cosv, const float *sinv,
+              const float *sino, float *img, float out_scale, uint32_t flags,
+              void *stream);
+
 /* ------------------------------------------------------ ray geometry */
 
 /* Ray endpoints of a circular orbit, generated on the device

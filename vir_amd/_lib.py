@@ -53,6 +53,18 @@ class VirSFGeom(ctypes.Structure):
 
 SG = ctypes.POINTER(VirSFGeom)
 
+
+class VirRDGeom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("nu", ctypes.c_int32), ("nv", ctypes.c_int32), ("na", ctypes.c_int32),
+                ("mode", ctypes.c_int32), ("joseph", ctypes.c_int32),
+                ("d_pix", ctypes.c_float), ("du", ctypes.c_float), ("dv", ctypes.c_float),
+                ("su", ctypes.c_float), ("sv", ctypes.c_float), ("DSO", ctypes.c_float),
+                ("DSD", ctypes.c_float)]
+
+
+RG = ctypes.POINTER(VirRDGeom)
+
 # name -> (restype, argtypes); the exported set of include/vir_amd.h
 SIGNATURES = {
     "vir_version": (ctypes.c_int, []),
@@ -69,6 +81,8 @@ SIGNATURES = {
     "vir_dd2d_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddcone_fp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddcone_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
+    "vir_rd_fp": (ctypes.c_int, [RG, P, P, P, P, F32, U32, P]),
+    "vir_rd_bp": (ctypes.c_int, [RG, P, P, P, P, F32, U32, P]),
     "vir_sf_fp": (ctypes.c_int, [SG, P, P, P, P, P, F32, U32, P]),
     "vir_sf_bp": (ctypes.c_int, [SG, P, P, P, P, P, F32, U32, P]),
     "vir_sirt_residual": (ctypes.c_int, [P, P, P, I64, P]),
