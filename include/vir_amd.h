@@ -161,6 +161,30 @@ int vir_ddcone_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
                   const float *sino, float *vol, float out_scale, uint32_t flags,
                   void *stream);
 
+/* --------------------------------------- square (translational) DD */
+
+/* Translational four-sided geometry (sys_mat/dd.py:903-1313): nsrc_p x
+ * nsrc_z source positions per side (censpace(nsrc, dsrc, ssrc)), detector
+ * boundspace(nu, du, su+ssrc_p) x boundspace(nv, dv, sv+ssrc_z), volume
+ * (n, n, nz) of voxel d_pix; sinogram (nsrc_p, nsrc_z, nv, nu, 4). */
+typedef struct vir_ddsq_geom {
+    int32_t n, nz;
+    int32_t nu, nv;
+    int32_t nsrc_p, nsrc_z;
+    float d_pix, du, dv, dsrc_p, dsrc_z, su, sv, ssrc_p, ssrc_z, DSO, DSD;
+} vir_ddsq_geom;
+
+/* sino = out_scale * sum of value * overlap_u * overlap_v over the four
+ * sides (dd_fp_square, dd.py:903-1098; the reference uses out_scale =
+ * d_pix^2/(du*dv)).  One thread per (source, bin), four sides each. */
+int vir_ddsq_fp(const vir_ddsq_geom *geom, const float *img, float *sino,
+                float out_scale, uint32_t flags, void *stream);
+
+/* Exact transpose of vir_ddsq_fp's sum (dd_bp_square, dd.py:1104-1313;
+ * the reference uses out_scale = 1).  One thread per voxel. */
+int vir_ddsq_bp(const vir_ddsq_geom *geom, const float *sino, float *img,
+                float out_scale, uint32_t flags, void *stream);
+
 /* ------------------------------------------ separable footprint (pixel-driven) */
 
 /* Geometry of the pixel-driven separable-footprint projectors

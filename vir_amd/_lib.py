@@ -65,6 +65,16 @@ class VirRDGeom(ctypes.Structure):
 
 RG = ctypes.POINTER(VirRDGeom)
 
+
+class VirDDSqGeom(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("nz", ctypes.c_int32), ("nu", ctypes.c_int32),
+                ("nv", ctypes.c_int32), ("nsrc_p", ctypes.c_int32), ("nsrc_z", ctypes.c_int32)] + \
+               [(k, ctypes.c_float) for k in ("d_pix", "du", "dv", "dsrc_p", "dsrc_z", "su", "sv",
+                                              "ssrc_p", "ssrc_z", "DSO", "DSD")]
+
+
+QG = ctypes.POINTER(VirDDSqGeom)
+
 # name -> (restype, argtypes); the exported set of include/vir_amd.h
 SIGNATURES = {
     "vir_version": (ctypes.c_int, []),
@@ -81,6 +91,8 @@ SIGNATURES = {
     "vir_dd2d_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddcone_fp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddcone_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
+    "vir_ddsq_fp": (ctypes.c_int, [QG, P, P, F32, U32, P]),
+    "vir_ddsq_bp": (ctypes.c_int, [QG, P, P, F32, U32, P]),
     "vir_rd_fp": (ctypes.c_int, [RG, P, P, P, P, F32, U32, P]),
     "vir_rd_bp": (ctypes.c_int, [RG, P, P, P, P, F32, U32, P]),
     "vir_sf_fp": (ctypes.c_int, [SG, P, P, P, P, P, F32, U32, P]),

@@ -11,6 +11,10 @@ output scaling:
                 d_pix=1.0, sample=1)                                       dd.py:815-825
   dd_bp_cone_3d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, dv=1.0, su=0.0,
                 sv=0.0, d_pix=1.0)                                         dd.py:829-835
+  dd_fp_square(img, nu, nv, nsrc_p, nsrc_z, DSO, DSD, du=1.0, dv=1.0, dsrc_p=1.0,
+               dsrc_z=1.0, su=0.0, sv=0.0, ssrc_p=0.0, ssrc_z=0.0, d_pix=1.0)  dd.py:903-930
+  dd_bp_square(sino, img_shape, DSO, DSD, du=1.0, dv=1.0, dsrc_p=1.0, dsrc_z=1.0,
+               su=0.0, sv=0.0, ssrc_p=0.0, ssrc_z=0.0, d_pix=1.0)          dd.py:1104-1135
 
 The reference's backprojectors are A^T scaled by du/(na*d_pix^2) (2-D) and
 du*dv/(na*d_pix^3) (cone); these functions keep that scaling.  DDProjector is
@@ -179,3 +183,69 @@ def dd_bp_cone_3d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv
     P = DDProjector(img_shape, ang_arr, sino.shape[1], "cone", nv=sino.shape[2], DSO=DSO, DSD=DSD,
                     du=du, dv=dv, su=su, sv=sv, d_pix=d_pix)
     return _wrap(P.bp(sino, reference_scale=True), torch_in)
+
+
+class SquareDDProjector:
+    """Distance-driven projector of the translational four-sided geometry
+    (dd.py:903-1313): fp = the reference's dd_fp_square (scaled by
+    d_pix^2/(du*dv)), bp = the reference's dd_bp_square (unscaled), which is
+    the transpose of the unscaled forward sum.  Volume (n, n, nz); sinogram
+    (nsrc_p, nsrc_z, nv, nu, 4)."""
+
+    def __init__(self, img_shape, nu, nv, nsrc_p, nsrc_z, DSO, DSD, du=1.0, dv=1.0, dsrc_p=1.0,
+                 dsrc_z=1.0, su=0.0, sv=0.0, ssrc_p=0.0, ssrc_z=0.0, d_pix=1.0):
+        require_gpu()
+        nx, ny, nz = (int(v) for v in img_shape)
+        if nx != ny:
+            raise ValueError("nx must equal ny")
+        self.img_shape = (nx, ny, nz)
+        self.sino_shape = (int(nsrc_p), int(nsrc_z), int(nv), int(nu), 4)
+        g = _lib.VirDDSqGeom()
+        g.n, g.nz, g.nu, g.nv, g.nsrc_p, g.nsrc_z = nx, nz, int(nu), int(nv), int(nsrc_p), int(nsrc_z)
+        for k, v in dict(d_pix=d_pix, du=du, dv=dv, dsrc_p=dsrc_p, dsrc_z=dsrc_z, su=su, sv=sv,
+                         ssrc_p=ssrc_p, ssrc_z=ssrc_z, DSO=DSO, DSD=DSD).items():
+            setattr(g, k, float(np.float32(v)))
+        self.g = g
+        d, du32, dv32 = np.float32(d_pix), np.float32(du), np.float32(dv)
+        self.fp_scale = float(d * d / (du32 * dv32))
+
+    def fp(self, img, out=None, accumulate=False):
+        img = as_device_f32(img)
+        if img.numel() != int(np.prod(self.img_shape)):
+            raise ValueError("image size does not match")
+        if out is None:
+            out = torch.empty(self.sino_shape, dtype=torch.float32, device=img.device)
+        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        _lib.check(_lib.lib().vir_ddsq_fp(self.g, _lib.ptr(img), _lib.ptr(out), self.fp_scale, fl,
+                                          stream()), "vir_ddsq_fp")
+        return out
+
+    def bp(self, sino, out=None, accumulate=False):
+        sino = as_device_f32(sino)
+        if sino.numel() != int(np.prod(self.sino_shape)):
+            raise ValueError("sinogram size does not match")
+        if out is None:
+            out = torch.empty(self.img_shape, dtype=torch.float32, device=sino.device)
+        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        _lib.check(_lib.lib().vir_ddsq_bp(self.g, _lib.ptr(sino), _lib.ptr(out), 1.0, fl, stream()),
+                   "vir_ddsq_bp")
+        return out
+
+
+def dd_fp_square(img, nu, nv, nsrc_p, nsrc_z, DSO, DSD, du=1.0, dv=1.0, dsrc_p=1.0, dsrc_z=1.0,
+                 su=0.0, sv=0.0, ssrc_p=0.0, ssrc_z=0.0, d_pix=1.0):
+    """dd.py:903-930: sinogram (nsrc_p, nsrc_z, nv, nu, 4) * d_pix^2/(du*dv)."""
+    torch_in = isinstance(img, torch.Tensor)
+    P = SquareDDProjector(tuple(img.shape), nu, nv, nsrc_p, nsrc_z, DSO, DSD, du, dv, dsrc_p,
+                          dsrc_z, su, sv, ssrc_p, ssrc_z, d_pix)
+    return _wrap(P.fp(img), torch_in)
+
+
+def dd_bp_square(sino, img_shape, DSO, DSD, du=1.0, dv=1.0, dsrc_p=1.0, dsrc_z=1.0, su=0.0, sv=0.0,
+                 ssrc_p=0.0, ssrc_z=0.0, d_pix=1.0):
+    """dd.py:1104-1135: the unscaled transpose."""
+    torch_in = isinstance(sino, torch.Tensor)
+    nsrc_p, nsrc_z, nv, nu = (int(v) for v in sino.shape[:4])
+    P = SquareDDProjector(img_shape, nu, nv, nsrc_p, nsrc_z, DSO, DSD, du, dv, dsrc_p, dsrc_z, su,
+                          sv, ssrc_p, ssrc_z, d_pix)
+    return _wrap(P.bp(sino), torch_in)
