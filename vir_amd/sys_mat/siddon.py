@@ -255,3 +255,11 @@ def csr_transpose(ind, ln, off, nvox):
                                             nvox, _lib.ptr(colptr), _lib.ptr(row), _lib.ptr(clen),
                                             stream()), "vir_csr_transpose")
     return colptr, row[:nnz], clen[:nnz]
+
+
+# list utilities of sys_mat/siddon.py:12-633, same module as in the reference
+from .siddon_lists import (list2array, list2array_iter_ray, list_ave, list_convole,  # noqa: E402,F401
+                           list_count_elem, list_del_ind, list_flatten, list_flip,
+                           list_index_shift, list_ravel, list_unravel, list_weight_ave, ray_boundary,
+                           ray_flip, ray_index_shift, ray_ravel, ray_unravel, rays_ave,
+                           rays_weight_ave)
