@@ -8,7 +8,7 @@ exact adjoint.  70,778,880 rays per step; inputs resident in HBM.
 
 Prints ONE JSON line (rank 0): value = rays/s = rays * steps / timed seconds,
 roofline of the dominant kernel from HIP-event kernel timings, and the CPU
-baseline (the C oracle port, single thread, on a bounded ray sample).
+baseline (the C oracle port on the host cores, on a bounded ray sample).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -62,35 +62,39 @@ def log(msg):
 
 
 def cpu_baseline(src, trg, grid, target_s):
-    """The reference's CPU path -- siddons() trace to CSR, CSR forward
-    projection and CSR backprojection -- as the C oracle port (single
-    thread), on a bounded contiguous sample of rays of the same workload."""
+    """The reference's CPU path -- siddons() trace, CSR forward projection
+    with a float64 accumulator (parallel over rays, as proj_numba's prange
+    path) and CSR backprojection -- as the C oracle port on every host core
+    this process may use (at most 16, the box's share), on a bounded
+    contiguous sample of rays of the same workload."""
     root = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, root)
-    from oracle import projection as op
     from oracle import siddon as osd
     nP, dP = grid["nPixels"], grid["dPixels"]
     s = src.reshape(-1, 3)
     t = trg.reshape(-1, 3)
     rng = np.random.default_rng(0)
     vol = rng.uniform(0, 1, nP).astype(np.float32)
-    nvox = int(np.prod(nP))
+    nth = max(1, min(16, len(os.sched_getaffinity(0))))
 
     def run(n, start):
         t0 = time.perf_counter()
-        ind, ln, off = osd.siddons(s[start:start + n], t[start:start + n], nP, dP, 0.0, flat=True)
-        sino = op.fp64(vol, ind, ln, off).astype(np.float32)
-        op.bp64(sino, ind, ln, off, nvox)
+        osd.fp_bp_threads(s[start:start + n], t[start:start + n], vol, nP, dP, nthreads=nth)
         return time.perf_counter() - t0
 
-    start = s.shape[0] // 2
-    n = 2048
+    start = s.shape[0] // 4
+    avail = s.shape[0] - start
+    n = 8192 * nth
     dt = run(n, start)
-    n = int(min(s.shape[0] - start, max(n, n * target_s / max(dt, 1e-3))))
-    dt = run(n, start)
-    return {"value": n / dt, "unit": "rays/s", "cores": 1, "kind": "port",
-            "sample": f"{n} consecutive rays of the same scan (view {start // (512 * 384)}), "
-                      f"C oracle siddons() trace + float64 CSR FP + BP, {dt:.1f} s"}
+    # grow the sample until it takes about target_s (fixed set-up costs make
+    # small samples look slow)
+    while dt < 0.6 * target_s and n < avail:
+        n = int(min(avail, max(2 * n, n * target_s / max(dt, 1e-3))))
+        dt = run(n, start)
+    return {"value": n / dt, "unit": "rays/s", "cores": nth, "kind": "port",
+            "sample": f"{n} consecutive rays of the same scan (from view {start // (512 * 384)}), "
+                      f"C oracle siddons() trace + float64 CSR FP + CSR BP on {nth} OpenMP "
+                      f"threads, {dt:.1f} s"}
 
 
 def events(n):

@@ -30,6 +30,10 @@ def load_library(path):
         raise RuntimeError(f"{path} missing: run `make -C oracle`")
     lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
+    if hasattr(lib, "oracle_fp_bp"):
+        lib.oracle_fp_bp.restype = ctypes.c_int64
+        lib.oracle_fp_bp.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_double, P, P, P,
+                                     ctypes.c_int]
     if True:
         lib.oracle_siddons.restype = ctypes.c_int64
         lib.oracle_siddons.argtypes = [ctypes.c_int64, P, P, P, P, P,
@@ -102,6 +106,25 @@ def trace(src, trg, nPixels=128, dPixels=1.0, origin=0.0, epsilon=1e-5, lib=None
     if tot < 0:
         raise RuntimeError("oracle capacity exceeded")
     return shape, x[:tot], y[:tot], z[:tot], ln[:tot], off, bad[:nr], n
+
+
+def fp_bp_threads(src, trg, vol, nPixels=128, dPixels=1.0, origin=0.0, epsilon=1e-5,
+                  nthreads=1):
+    """The reference's CPU path on `nthreads` OpenMP threads: trace, float64
+    CSR forward projection, float32 backprojection of that sinogram.
+    Returns (sino float32 (nrays,), back float32 nPixels, nnz)."""
+    lib = _load()
+    n, d, o = grid_params(nPixels, dPixels, origin)
+    src = np.ascontiguousarray(np.asarray(src, np.float32).reshape(-1, 3))
+    trg = np.ascontiguousarray(np.asarray(trg, np.float32).reshape(-1, 3))
+    vol = np.ascontiguousarray(np.asarray(vol, np.float32).reshape(tuple(int(v) for v in n)))
+    sino = np.zeros(src.shape[0], np.float32)
+    back = np.zeros(vol.shape, np.float32)
+    nnz = lib.oracle_fp_bp(src.shape[0], _ptr(src), _ptr(trg), _ptr(n), _ptr(d), _ptr(o),
+                           float(epsilon), _ptr(vol), _ptr(sino), _ptr(back), int(nthreads))
+    if nnz < 0:
+        raise RuntimeError("oracle_fp_bp failed")
+    return sino, back, int(nnz)
 
 
 def siddons(src, trg, nPixels=128, dPixels=1.0, origin=0.0, ravel=False,

@@ -32,6 +32,9 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <string.h>
 
 #define RTOL_F 1e-5f
@@ -250,4 +253,83 @@ int64_t oracle_siddons(int64_t nrays, const float *src, const float *trg,
 /* Upper bound on entries per ray (planes + 2 bounds - 1 segments). */
 int64_t oracle_siddons_maxlen(const int64_t n[3]) {
     return (n[0] + 1) + (n[1] + 1) + (n[2] + 1) + 1;
+}
+
+/*
+ * CPU baseline of the reference's CPU path for one batch of rays: the
+ * siddons() trace (this restatement), the CSR forward projection with a
+ * float64 accumulator (projection/proj_numba.py:11-32, parallel over rays
+ * like fast_forward_project_p) and the CSR backprojection (float32,
+ * projection/projection_c.c:176-219), over `nthreads` OpenMP threads.  The
+ * backprojection goes into per-thread volumes summed at the end.  Rays are
+ * traced in chunks so each call of oracle_siddons amortises its set-up.
+ * Returns the number of (ray, voxel) entries, or -1 on allocation failure.
+ */
+int64_t oracle_fp_bp(int64_t nrays, const float *src, const float *trg,
+                     const int64_t n[3], const float d[3], const float o[3], double epsilon,
+                     const float *vol, float *sino, float *back, int nthreads) {
+    const int64_t nvox = n[0] * n[1] * n[2];
+    const int64_t maxlen = oracle_siddons_maxlen(n);
+    const int64_t chunk = 256;
+    if (nthreads < 1) nthreads = 1;
+    float *part = (float *)calloc((size_t)nthreads * (size_t)nvox, sizeof(float));
+    if (!part) return -1;
+    int64_t total = 0;
+    int failed = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : total)
+    {
+#ifdef _OPENMP
+        const int tid = omp_get_thread_num();
+#else
+        const int tid = 0;
+#endif
+        float *mine = part + (size_t)tid * (size_t)nvox;
+        const int64_t cap = chunk * maxlen;
+        int32_t *x = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+        int32_t *y = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+        int32_t *z = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+        float *ln = (float *)malloc(sizeof(float) * (size_t)cap);
+        int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(chunk + 1));
+        uint8_t *bad = (uint8_t *)malloc((size_t)chunk);
+        if (!x || !y || !z || !ln || !off || !bad) {
+#pragma omp atomic write
+            failed = 1;
+        } else {
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t r0 = 0; r0 < nrays; r0 += chunk) {
+                const int64_t cnt = r0 + chunk <= nrays ? chunk : nrays - r0;
+                int64_t m = oracle_siddons(cnt, src + 3 * r0, trg + 3 * r0, n, d, o, epsilon, x, y, z, ln,
+                                           off, bad, cap);
+                if (m < 0) continue;
+                total += m;
+                for (int64_t k = 0; k < cnt; ++k) {
+                    double s = 0.0;
+                    for (int64_t e = off[k]; e < off[k + 1]; ++e) {
+                        if (x[e] < 0 || y[e] < 0 || z[e] < 0 || x[e] >= n[0] || y[e] >= n[1] || z[e] >= n[2])
+                            continue;
+                        const int64_t vi = ((int64_t)x[e] * n[1] + y[e]) * n[2] + z[e];
+                        s += (double)vol[vi] * (double)ln[e];
+                    }
+                    const float sv = (float)s;
+                    sino[r0 + k] = sv;
+                    for (int64_t e = off[k]; e < off[k + 1]; ++e) {
+                        if (x[e] < 0 || y[e] < 0 || z[e] < 0 || x[e] >= n[0] || y[e] >= n[1] || z[e] >= n[2])
+                            continue;
+                        mine[((int64_t)x[e] * n[1] + y[e]) * n[2] + z[e]] += sv * ln[e];
+                    }
+                }
+            }
+        }
+        free(x); free(y); free(z); free(ln); free(off); free(bad);
+    }
+    if (!failed) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t v = 0; v < nvox; ++v) {
+            float s = 0.0f;
+            for (int t = 0; t < nthreads; ++t) s += part[(size_t)t * (size_t)nvox + v];
+            back[v] = s;
+        }
+    }
+    free(part);
+    return failed ? -1 : total;
 }

@@ -162,3 +162,24 @@ def test_golden_vectors_catch_planted_mistakes(golden, tmp_path, plant):
               np.array_equal(ln.view(np.int32), golden[key + "_flen"].view(np.int32)))
         mismatches += not ok
     assert mismatches > 0
+
+
+def test_threaded_cpu_path_matches_trace_products():
+    """The multi-threaded CPU baseline (trace + float64 FP + float32 BP) equals
+    the sparse products of the single-threaded trace."""
+    from oracle import projection as op
+    from vir_amd.proj_geom import geom_circular
+    views = np.arange(6) * np.pi / 3 + 0.2
+    s, t = geom_circular((np.arange(20) - 9.5) * 1.1, views, "cone", src_iso=40.0, det_iso=30.0,
+                         DetsZ=(np.arange(10) - 4.5) * 1.1)
+    nP = (12, 11, 9)
+    rng = np.random.default_rng(0)
+    vol = rng.uniform(0, 1, nP).astype(np.float32)
+    ind, ln, off = osd.siddons(s, t, nP, 0.9, 0.0, flat=True)
+    ref_fp = op.fp64(vol, ind, ln, off).astype(np.float32)
+    ref_bp = op.bp64(ref_fp, ind, ln, off, int(np.prod(nP)))
+    for nth in (1, 4):
+        sino, back, nnz = osd.fp_bp_threads(s, t, vol, nP, 0.9, nthreads=nth)
+        assert nnz == ind.size
+        assert np.array_equal(sino, ref_fp)
+        np.testing.assert_allclose(back.reshape(-1), ref_bp.reshape(-1), rtol=1e-5, atol=1e-5)
