@@ -243,8 +243,12 @@ class _Job:
             if graph:
                 gr = torch.cuda.CUDAGraph()
                 gr.capture_begin()
-                S._iteration()
-                gr.capture_end()
+                try:
+                    S._iteration()
+                finally:
+                    # close the capture even when a launch fails, so the
+                    # stream stays usable for the next request
+                    gr.capture_end()
                 self.graph = gr
                 for _ in range(req.iters):
                     gr.replay()
