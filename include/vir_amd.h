@@ -299,6 +299,22 @@ int vir_tv(const float *x, int64_t nbatch, int32_t nx, int32_t ny, int32_t nz,
            float eps, float weight, float *grad, double *value, double *work,
            uint32_t flags, void *stream);
 
+/* Chambolle-Pock iteration for TV-regularised reconstruction (tv.py:229-294),
+ * on volumes (nx, ny, nz) with periodic differences (tv.py:207-224); nz == 1
+ * is 2-D.  z holds ndim (2 or 3) planes of the volume's size.
+ *   dual TV:   z_c += sigma_z * (roll(ubar,-1,c) - ubar); z /= max(1, |z|)
+ *              (the unit-ball projection pyT2, tv.py:24-95, intends)
+ *   dual data: q = (q + sigma_q*(Au - g)) / (1 + sigma_q*lam)      (tv.py:283-284)
+ *   primal:    u' = u + tau*(div z - ATq); ubar = u' + theta*(u' - u)
+ *                                                                   (tv.py:287-291) */
+int vir_pdhg_dual_tv(int32_t nx, int32_t ny, int32_t nz, const float *ubar,
+                     float *z, float sigma_z, void *stream);
+int vir_pdhg_dual_data(int64_t n, const float *Au, const float *g,
+                       const float *sigma_q, float lam, float *q, void *stream);
+int vir_pdhg_primal(int32_t nx, int32_t ny, int32_t nz, const float *z,
+                    const float *ATq, const float *tau, float theta, float *u,
+                    float *ubar, void *stream);
+
 /* Least-squares data term: r = a - b (r may be NULL) and
  * value = 0.5 * sum (a - b)^2 (one device double, may be NULL; needs `work`
  * of VIR_TV_WORK doubles), summed in double in a fixed order.  No reference

@@ -26,6 +26,15 @@ bin) overlap integral, and the FP/BP scaling relation.
 import numpy as np
 
 
+def _trig(ang):
+    """cos/sin per view as the reference evaluates them: np.cos / np.sin of
+    the float32 angle array (as_float32 dd.py:29-35, dd.py:702-703), widened
+    to float64.  At a diagonal view |sin| == |cos| holds only in float32,
+    and that tie fixes the driving axis."""
+    a32 = np.asarray(ang, np.float32).reshape(-1)
+    return np.cos(a32).astype(np.float64), np.sin(a32).astype(np.float64)
+
+
 def _bnd(n, d):
     return d * (np.arange(n + 1, dtype=np.float64) - n / 2.0)
 
@@ -111,8 +120,9 @@ def par_2d(img_or_sino, ang, nu_or_shape, du=1.0, su=0.0, d_pix=1.0, bp=False):
         img = np.asarray(img_or_sino, np.float64)
         nx, ny = img.shape
         sino = np.zeros((ang.size, nu_or_shape))
-    for ia, a in enumerate(ang):
-        c, s = np.cos(a), np.sin(a)
+    trig_c, trig_s = _trig(ang)
+    for ia in range(ang.size):
+        c, s = trig_c[ia], trig_s[ia]
         xd, pdrv, porth, scl, rev = par_geom(nx, ny, d_pix, c, s)
         if bp:
             tgt = img_x if xd else img_y
@@ -150,8 +160,9 @@ def fan_2d(img_or_sino, ang, nu_or_shape, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, b
         nx, ny = img.shape
         ub = _ubnd(nu_or_shape, du, su)
         sino = np.zeros((ang.size, nu_or_shape))
-    for ia, a in enumerate(ang):
-        c, s = np.cos(a), np.sin(a)
+    trig_c, trig_s = _trig(ang)
+    for ia in range(ang.size):
+        c, s = trig_c[ia], trig_s[ia]
         xd, pdrv, porth, odrv, oorth, scl, rev = fan_geom(nx, ny, d_pix, c, s, DSO, DSD)
         if bp:
             tgt = img_x if xd else img_y
@@ -181,8 +192,9 @@ def cone_fp(img, ang, nu, nv, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=1.0, d_pix=1.
     ub, vb = _ubnd(nu, du, su), _ubnd(nv, dv, sv)
     u0, v0 = ub[0], vb[0]
     sino = np.zeros((ang.size, nu, nv))
-    for ia, a in enumerate(ang):
-        c, s = np.cos(a), np.sin(a)
+    trig_c, trig_s = _trig(ang)
+    for ia in range(ang.size):
+        c, s = trig_c[ia], trig_s[ia]
         xd, pdrv, porth, odrv, oorth, scl, rev = fan_geom(nx, ny, d_pix, c, s, DSO, DSD, True)
         vol = img if xd else img.transpose(1, 0, 2)
         if rev:
@@ -245,8 +257,9 @@ def cone_bp(sino, ang, img_shape, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=0.0, d_pi
     v0 = vb[0]
     img_x = np.zeros((nx, ny, nz))
     img_y = np.zeros((ny, nx, nz))
-    for ia, a in enumerate(ang):
-        c, s = np.cos(a), np.sin(a)
+    trig_c, trig_s = _trig(ang)
+    for ia in range(ang.size):
+        c, s = trig_c[ia], trig_s[ia]
         xd, pdrv, porth, odrv, oorth, scl, rev = fan_geom(nx, ny, d_pix, c, s, DSO, DSD, True)
         tgt = img_x if xd else img_y
         vol = tgt[::-1] if rev else tgt
@@ -286,14 +299,16 @@ def cone_bp(sino, ang, img_shape, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=0.0, d_pi
 def overlap_brute_2d(nx, ny, ang, nu, du, su, d_pix, geom="par", DSO=None, DSD=None, sub=64):
     """Independent check: system matrix of the DD model by direct evaluation
     of each pixel's projected driving-axis segment against each bin (no
-    sweep), float64.  Returns A of shape (na*nu, nx*ny) with the FP scaling."""
+    sweep), float64.  Returns A of shape (na*nu, nx*ny) with the FP scaling;
+    cos/sin per view from _trig, as the sweeps."""
     ang = np.asarray(ang, np.float64)
+    trig_c, trig_s = _trig(ang)
     A = np.zeros((ang.size, nu, nx, ny))
     ub = _ubnd(nu, du, su)
     xb, yb = _bnd(nx, d_pix), _bnd(ny, d_pix)
     xc, yc = _cen(xb), _cen(yb)
-    for ia, a in enumerate(ang):
-        c, s = np.cos(a), np.sin(a)
+    for ia in range(ang.size):
+        c, s = trig_c[ia], trig_s[ia]
         xdrv = abs(s) >= abs(c)
         for ix in range(nx):
             for iy in range(ny):

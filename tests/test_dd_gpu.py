@@ -62,7 +62,7 @@ def test_dd_fan_2d(detector):
     if detector == "flat":
         r = dd_ref.fan_2d(img, ang, nu, DSO, DSD, du=1.1, su=0.2, d_pix=0.9)
     else:
-        A = dd_ref.overlap_brute_2d(nx, ny, ang.astype(np.float64), nu, 1.1, 0.2, 0.9, geom="arc",
+        A = dd_ref.overlap_brute_2d(nx, ny, ang, nu, 1.1, 0.2, 0.9, geom="arc",
                                     DSO=DSO, DSD=DSD)
         r = (A @ img.reshape(-1).astype(np.float64)).reshape(na, nu)
     assert rel(g, r) <= TOL
@@ -71,6 +71,26 @@ def test_dd_fan_2d(detector):
         gb = dd.dd_bp_fan_2d(y, ang, (nx, ny), DSO, DSD, du=1.1, su=0.2, d_pix=0.9)
         rb = dd_ref.fan_2d(y, ang, (nx, ny), DSO, DSD, du=1.1, su=0.2, d_pix=0.9, bp=True)
         assert rel(gb, rb) <= TOL
+
+
+def test_dd_fan_2d_diagonal_views_narrow_detector():
+    """Views at exact multiples of 45 degrees (float32 angles, where
+    |sin| == |cos| decides the driving axis as in the reference's float32
+    np.cos) and a detector narrower than the image's shadow."""
+    from vir_amd.sys_mat import dd
+    rng = np.random.default_rng(5)
+    n, nu, na, du, DSO, DSD = 20, 24, 24, 1.2, 50.0, 80.0
+    img = _phantom((n, n), rng)
+    ang = np.linspace(0, 2 * np.pi, na, endpoint=False).astype(np.float32)
+    g = dd.dd_fp_fan_2d(img, ang, nu, DSO, DSD, du=du)
+    r = dd_ref.fan_2d(img, ang, nu, DSO, DSD, du=du)
+    A = dd_ref.overlap_brute_2d(n, n, ang, nu, du, 0.0, 1.0, geom="fan", DSO=DSO, DSD=DSD)
+    assert rel(r, A @ img.reshape(-1).astype(np.float64)) < 1e-12
+    assert rel(g, r) <= TOL
+    y = rng.uniform(0, 1, r.shape).astype(np.float32)
+    gb = dd.dd_bp_fan_2d(y, ang, (n, n), DSO, DSD, du=du)
+    rb = dd_ref.fan_2d(y, ang, (n, n), DSO, DSD, du=du, bp=True)
+    assert rel(gb, rb) <= TOL
 
 
 def test_dd_cone_3d():

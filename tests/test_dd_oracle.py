@@ -34,17 +34,21 @@ def test_impulse_response():
     img[3, 5] = 1.0
     for a, (cs, sn) in [(0.0, (1.0, 0.0)), (np.pi / 2, (0.0, 1.0))]:
         sino = dd_ref.par_2d(img, np.array([a]), 40, du=0.25, d_pix=1.0)[0]
-        # one pixel covers d_pix of detector length; height d_pix/du * scale
-        assert abs(sino.sum() * 0.25 - 1.0) < 1e-12
-        nz = sino[sino > 1e-12]
-        assert len(nz) == 4 and np.allclose(nz, 1.0 / 0.25 * 0.25)
+        # one pixel covers d_pix of detector length; height d_pix/du * scale.
+        # cos/sin come from the float32 angle (cos(float32(pi/2)) = -4.4e-8),
+        # so a sliver of order 1e-7 may fall into a fifth bin.
+        assert abs(sino.sum() * 0.25 - 1.0) < 1e-6
+        nz = sino[sino > 1e-5]
+        assert len(nz) == 4 and np.allclose(nz, 1.0 / 0.25 * 0.25, atol=1e-6)
 
 
 @pytest.mark.parametrize("geom", ["par", "fan"])
 def test_sweep_equals_brute_force(geom):
     rng = np.random.default_rng(1)
     nx, ny, nu = 6, 5, 23
-    ang = rng.uniform(0, 2 * np.pi, 7)
+    # random views plus exact diagonals, where the float32 |sin| == |cos|
+    # tie picks the driving axis
+    ang = np.concatenate([rng.uniform(0, 2 * np.pi, 7), np.arange(1, 8, 2) * np.pi / 4])
     du, su, d_pix = 0.9, 0.25, 1.0
     kw = dict(DSO=30.0, DSD=45.0) if geom == "fan" else {}
     A = dd_ref.overlap_brute_2d(nx, ny, ang, nu, du, su, d_pix, geom=geom, **kw)

@@ -83,7 +83,7 @@ def test_sirt_dd_fan_and_graph_replay():
     nu, na = 40, 30
     ang = np.linspace(0, 2 * np.pi, na, endpoint=False).astype(np.float32)
     DSO, DSD, du = 60.0, 90.0, 1.2
-    A = dd_ref.overlap_brute_2d(nx, ny, ang.astype(np.float64), nu, du, 0.0, 1.0, geom="fan",
+    A = dd_ref.overlap_brute_2d(nx, ny, ang, nu, du, 0.0, 1.0, geom="fan",
                                 DSO=DSO, DSD=DSD)
     rng = np.random.default_rng(2)
     x_true = np.zeros((nx, ny), np.float32)

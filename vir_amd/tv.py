@@ -60,3 +60,53 @@ def denoise_tv_chambolle(image, weight=0.1, eps=2.0e-4, max_num_iter=200):
                 break
             e_prev = e
     return out
+
+
+def _shape3(shape):
+    s = tuple(int(v) for v in shape)
+    if len(s) == 2:
+        return s + (1,)
+    if len(s) == 3:
+        return s
+    raise ValueError("pdhg_tv needs a 2-D or 3-D image")
+
+
+def pdhg_tv(P, g, iters=100, lam=1.0, theta=1.0, x0=None):
+    """TV-regularised reconstruction by the Chambolle-Pock primal-dual
+    iteration of tv.py:229-294 with diagonal step sizes: sigma_z = 1/2
+    (1/grad_abs(1)), sigma_q = 1/(A 1) (0 where A 1 = 0), tau = 1/(A^T 1 +
+    div_abs(1)) = 1/(A^T 1 + 2*ndim).  P: any projector with fp/bp on device
+    tensors; g: the sinogram.  Returns u (device)."""
+    from . import _lib
+    from .device import as_device_f32, stream
+    img_shape = tuple(getattr(P, "img_shape", None) or getattr(P, "nPixels"))
+    sino_shape = tuple(getattr(P, "sino_shape", None) or getattr(P, "ray_shape"))
+    nx, ny, nz = _shape3(img_shape if len(img_shape) != 3 or img_shape[2] != 1 else img_shape[:2])
+    ndim = 3 if nz > 1 else 2
+    g = as_device_f32(g).reshape(sino_shape)
+    dev = g.device
+    ones_i = torch.ones(img_shape, dtype=torch.float32, device=dev)
+    ones_s = torch.ones(sino_shape, dtype=torch.float32, device=dev)
+    A1 = P.fp(ones_i)
+    sigma_q = torch.where(A1 != 0, 1.0 / A1, torch.zeros_like(A1))
+    tau = 1.0 / (P.bp(ones_s) + 2.0 * ndim)
+    u = (torch.zeros(img_shape, dtype=torch.float32, device=dev) if x0 is None
+         else as_device_f32(x0).reshape(img_shape).clone())
+    ubar = u.clone()
+    z = torch.zeros((ndim,) + img_shape, dtype=torch.float32, device=dev)
+    q = torch.zeros(sino_shape, dtype=torch.float32, device=dev)
+    Au = torch.empty(sino_shape, dtype=torch.float32, device=dev)
+    ATq = torch.empty(img_shape, dtype=torch.float32, device=dev)
+    L = _lib.lib()
+    for _ in range(int(iters)):
+        st = stream()
+        _lib.check(L.vir_pdhg_dual_tv(nx, ny, nz, _lib.ptr(ubar), _lib.ptr(z), 0.5, st),
+                   "vir_pdhg_dual_tv")
+        P.fp(ubar, out=Au)
+        _lib.check(L.vir_pdhg_dual_data(q.numel(), _lib.ptr(Au), _lib.ptr(g), _lib.ptr(sigma_q),
+                                        float(lam), _lib.ptr(q), st), "vir_pdhg_dual_data")
+        P.bp(q, out=ATq)
+        _lib.check(L.vir_pdhg_primal(nx, ny, nz, _lib.ptr(z), _lib.ptr(ATq), _lib.ptr(tau),
+                                     float(theta), _lib.ptr(u), _lib.ptr(ubar), st),
+                   "vir_pdhg_primal")
+    return u
