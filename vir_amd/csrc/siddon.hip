@@ -33,11 +33,14 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_count(GridK g, const flo
         fl = 2;
         Walker W;
         walker_begin(g, R, W);
-        walk_entries<P2, ISO>(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float) {
-            ++cnt;
-            if ((unsigned)ix >= (unsigned)g.n[0] || (unsigned)iy >= (unsigned)g.n[1] ||
-                (unsigned)iz >= (unsigned)g.n[2])
-                fl |= 1;
+        walk_dispatch(g, R, [&](auto fast) {
+            walk_entries<P2, ISO, false, decltype(fast)::value>(
+                g, R, W, VIR_INF, [&](int ix, int iy, int iz, float) {
+                    ++cnt;
+                    if ((unsigned)ix >= (unsigned)g.n[0] || (unsigned)iy >= (unsigned)g.n[1] ||
+                        (unsigned)iz >= (unsigned)g.n[2])
+                        fl |= 1;
+                });
         });
     }
     counts[r] = cnt;
@@ -61,25 +64,28 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fill(GridK g, const floa
     const int64_t e_end = off[r + 1];
     Walker W;
     walker_begin(g, R, W);
-    walk_entries<P2, ISO, true>(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
-        if (e >= e_end) return;  // count and fill walk identically; never past the row
-        if (rav) {
-            bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
-                      (unsigned)iz < (unsigned)g.n[2];
-            rav[e] = in ? (ix * g.n[1] + iy) * g.n[2] + iz : -1;
-        }
-        if (ox) ox[e] = ix;
-        if (oy) oy[e] = iy;
-        if (oz) oz[e] = iz;
-        if (olen) olen[e] = len;
-        ++e;
+    walk_dispatch(g, R, [&](auto fast) {
+        walk_entries<P2, ISO, true, decltype(fast)::value>(
+            g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
+                if (e >= e_end) return;  // count and fill walk identically; never past the row
+                if (rav) {
+                    bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
+                              (unsigned)iz < (unsigned)g.n[2];
+                    rav[e] = in ? (ix * g.n[1] + iy) * g.n[2] + iz : -1;
+                }
+                if (ox) ox[e] = ix;
+                if (oy) oy[e] = iy;
+                if (oz) oz[e] = iz;
+                if (olen) olen[e] = len;
+                ++e;
+            });
     });
 }
 
 // ------------------------------------------------------------ forward
 
 template <bool P2, bool ISO>
-__global__ __launch_bounds__(kRayThreads) void k_siddon_fp(GridK g, const float* __restrict__ src,
+__global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_siddon_fp(GridK g, const float* __restrict__ src,
                                                           const float* __restrict__ trg, int64_t nrays,
                                                           const float* __restrict__ vol,
                                                           float* __restrict__ sino, int accumulate,
@@ -99,15 +105,18 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fp(GridK g, const float*
         Walker W;
         walker_begin(g, R, W);
         const unsigned nyz = (unsigned)(g.n[1] * g.n[2]);
-        walk_entries<P2, ISO>(g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
-            bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
-                      (unsigned)iz < (unsigned)g.n[2];
-            bad += in ? 0 : 1;
-            unsigned vi = (unsigned)ix * nyz + (unsigned)iy * (unsigned)g.n[2] + (unsigned)iz;
-            float v = in ? vol[vi] : 0.0f;
-            sum += (double)(pv * pl);
-            pv = v;
-            pl = in ? len : 0.0f;
+        walk_dispatch(g, R, [&](auto fast) {
+            walk_entries<P2, ISO, false, decltype(fast)::value>(
+                g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
+                    bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
+                              (unsigned)iz < (unsigned)g.n[2];
+                    bad += in ? 0 : 1;
+                    unsigned vi = (unsigned)ix * nyz + (unsigned)iy * (unsigned)g.n[2] + (unsigned)iz;
+                    float v = in ? vol[vi] : 0.0f;
+                    sum += (double)(pv * pl);
+                    pv = v;
+                    pl = in ? len : 0.0f;
+                });
         });
     }
     sum += (double)(pv * pl);
@@ -290,14 +299,18 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     if (miss || alo > ahi) continue;
                     // exact plane slices only matter near the ray's ends
                     if (!window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
-                    Walker W;
-                    walker_seek(g, R, W, alo);
-                    walk_entries<P2, ISO>(g, R, W, ahi, [&](int ix, int iy, int iz, float len) {
-                        int lx = ix - b0[0], ly = iy - b0[1], lz = iz - b0[2];
-                        if ((unsigned)lx < (unsigned)(b1[0] - b0[0]) && (unsigned)ly < (unsigned)(b1[1] - b0[1]) &&
-                            (unsigned)lz < (unsigned)(b1[2] - b0[2])) {
-                            acc[(lx * SBY + ly) * SBZ + lz] += wgt * len;
-                        }
+                    walk_dispatch(g, R, [&](auto fast) {
+                        constexpr bool F = decltype(fast)::value;
+                        Walker W;
+                        walker_seek<F>(g, R, W, alo);
+                        walk_entries<P2, ISO, false, F>(g, R, W, ahi, [&](int ix, int iy, int iz, float len) {
+                            int lx = ix - b0[0], ly = iy - b0[1], lz = iz - b0[2];
+                            if ((unsigned)lx < (unsigned)(b1[0] - b0[0]) &&
+                                (unsigned)ly < (unsigned)(b1[1] - b0[1]) &&
+                                (unsigned)lz < (unsigned)(b1[2] - b0[2])) {
+                                acc[(lx * SBY + ly) * SBZ + lz] += wgt * len;
+                            }
+                        });
                     });
                 }
                 // next colour phase: keep the compiler from moving LDS accesses

@@ -62,6 +62,19 @@ __device__ __forceinline__ float div_rn(float a, float b, float y) {
     return qq;
 }
 
+// div_rn without the range check, for operands the caller has shown to lie
+// inside the guard band (see planes_unguarded).
+__device__ __forceinline__ float div_rn_unguarded(float a, float b, float y) {
+    float q = a * y;
+    float r = __builtin_fmaf(-q, b, a);
+    return __builtin_fmaf(r, y, q);
+}
+
+template <bool FAST>
+__device__ __forceinline__ float div_rn_t(float a, float b, float y) {
+    return FAST ? div_rn_unguarded(a, b, y) : div_rn(a, b, y);
+}
+
 __device__ __forceinline__ float fmin_nn(float a, float b) { return b < a ? b : a; }
 
 // boundspace(n, d, c, float32) (core/core_funcs.py:62-81):
@@ -192,6 +205,48 @@ __device__ __forceinline__ void ray_setup(const GridK& g, const float* __restric
     ray_setup_slices(g, R);
 }
 
+// True when every plane alpha of the ray, RN((plane - s) / dST) over the
+// planes p0..pN of each axis the ray crosses, lies inside div_rn's guard
+// band, so the unguarded division returns the same bits.  With gap = the
+// smallest non-zero |plane - s| (planes are monotone in k: outside the span
+// it is the distance to the span, inside it one of the two planes bracketing
+// s, found among k-1..k+2 around the f32 estimate k), every numerator is 0
+// or of magnitude >= RN(gap); gap >= 2^-90, gap |1/dST| >= 2^-95 and
+// max |plane - s| |1/dST| <= 2^95 keep the quotient inside the band.  Rays
+// with a zero direction component (float64 lists) are excluded, as are
+// grids whose planes are not distinct in f32.
+__device__ __forceinline__ bool planes_unguarded(const GridK& g, const Ray& R) {
+    bool ok = !R.f64;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float s = R.s[a];
+        float gap;
+        if (s < g.p0[a]) {
+            gap = g.p0[a] - s;
+        } else if (s > g.pN[a]) {
+            gap = s - g.pN[a];
+        } else {
+            int k = (int)floorf((s - g.p0[a]) * g.inv_d[a]);
+            gap = VIR_INF;
+            float prev = -VIR_INF;
+#pragma unroll
+            for (int j = -1; j <= 2; ++j) {
+                int kk = k + j;
+                if (kk < 0 || kk > g.n[a]) continue;
+                float pc = plane_coord(g, a, kk);
+                ok = ok && pc > prev;  // distinct, increasing planes
+                prev = pc;
+                float dd = fabsf(pc - s);
+                if (dd > 0.0f) gap = fminf(gap, dd);
+            }
+        }
+        float hi = fmaxf(fabsf(g.p0[a] - s), fabsf(g.pN[a] - s));
+        float ay = fabsf(R.y[a]);
+        ok = ok && gap >= 0x1p-90f && gap * ay >= 0x1p-95f && hi * ay <= 0x1p95f && hi <= 0x1p100f;
+    }
+    return ok;
+}
+
 // True when the alpha window [t0, t1] lies at least three planes inside the
 // ray's entry and exit on every axis it crosses.  The snapped slice bounds
 // differ from the geometric entry/exit planes by at most one plane, so a walk
@@ -212,8 +267,9 @@ __device__ __forceinline__ bool window_interior(const GridK& g, const Ray& R, fl
     return ok;
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ float plane_alpha(const GridK& g, const Ray& R, int a, int k) {
-    return div_rn(plane_coord(g, a, k) - R.s[a], R.dST[a], R.y[a]);
+    return div_rn_t<FAST>(plane_coord(g, a, k) - R.s[a], R.dST[a], R.y[a]);
 }
 
 template <class T>
@@ -262,7 +318,7 @@ __device__ __forceinline__ void walker_begin(const GridK& g, const Ray& R, Walke
 // Branch-free in the stream choice: one division per step whichever axis
 // advances.  ISO: all three axes share (n, d, origin), so the plane
 // coordinate needs no per-axis selection.
-template <bool ISO>
+template <bool ISO, bool FAST>
 __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker& W) {
     float h0 = W.h[0], h1 = W.h[1], h2 = W.h[2];
     bool c1 = h1 < h0;
@@ -290,7 +346,7 @@ __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker
     float sa = s2 ? R.s[2] : (s1 ? R.s[1] : R.s[0]);
     float da = s2 ? R.dST[2] : (s1 ? R.dST[1] : R.dST[0]);
     float ya = s2 ? R.y[2] : (s1 ? R.y[1] : R.y[0]);
-    float hn = div_rn(pc - sa, da, ya);
+    float hn = div_rn_t<FAST>(pc - sa, da, ya);
     hn = k != ks ? hn : VIR_INF;
     W.kn[0] = s0 ? k : W.kn[0];
     W.kn[1] = s1 ? k : W.kn[1];
@@ -304,6 +360,7 @@ __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker
 // Position all streams just after every element <= T (used to enter a ray
 // part-way, at the guard plane of a volume brick).  Exact: each candidate
 // plane's alpha is recomputed by the same division and compared to T.
+template <bool FAST = false>
 __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker& W, float T) {
     W.remaining = 2;
     int cnt = 0;
@@ -323,14 +380,14 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
             // first k in [lo, hi) with alpha(k) > T
             k = fk < -1.0e9f ? lo : (fk > 1.0e9f ? hi : (int)fk + 1);
             k = k < lo ? lo : (k > hi ? hi : k);
-            while (k > lo && plane_alpha(g, R, a, k - 1) > T) --k;
-            hk = k < hi ? plane_alpha(g, R, a, k) : VIR_INF;
+            while (k > lo && plane_alpha<FAST>(g, R, a, k - 1) > T) --k;
+            hk = k < hi ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             while (k < hi && hk <= T) {
                 ++k;
-                hk = k < hi ? plane_alpha(g, R, a, k) : VIR_INF;
+                hk = k < hi ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             }
             int taken = k - lo;
-            if (taken > 0) pv = fmaxf(pv, plane_alpha(g, R, a, k - 1));
+            if (taken > 0) pv = fmaxf(pv, plane_alpha<FAST>(g, R, a, k - 1));
             cnt += taken;
             W.kn[a] = k; W.kstop[a] = hi; W.kdir[a] = 1;
             W.remaining += hi - k;
@@ -338,14 +395,14 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
             // stream runs hi-1 .. lo; next = largest k in [lo, hi) with alpha(k) > T
             k = fk < -1.0e9f ? lo - 1 : (fk > 1.0e9f ? hi - 1 : (int)fk);
             k = k < lo - 1 ? lo - 1 : (k > hi - 1 ? hi - 1 : k);
-            while (k < hi - 1 && plane_alpha(g, R, a, k + 1) > T) ++k;
-            hk = k >= lo ? plane_alpha(g, R, a, k) : VIR_INF;
+            while (k < hi - 1 && plane_alpha<FAST>(g, R, a, k + 1) > T) ++k;
+            hk = k >= lo ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             while (k >= lo && hk <= T) {
                 --k;
-                hk = k >= lo ? plane_alpha(g, R, a, k) : VIR_INF;
+                hk = k >= lo ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             }
             int taken = (hi - 1) - k;
-            if (taken > 0) pv = fmaxf(pv, plane_alpha(g, R, a, k + 1));
+            if (taken > 0) pv = fmaxf(pv, plane_alpha<FAST>(g, R, a, k + 1));
             cnt += taken;
             W.kn[a] = k; W.kstop[a] = lo - 1; W.kdir[a] = -1;
             W.remaining += k - (lo - 1);
@@ -364,14 +421,14 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
 // siddon.py:796-809.  Returns false when the reference drops the entry
 // (length not > 0).  idx holds the truncated index per axis.  P2: every
 // voxel size is a power of two, so dividing by it is a multiplication.
-template <bool P2, bool EXACT>
+template <bool P2, bool EXACT, bool FAST = false>
 __device__ __forceinline__ bool segment(const GridK& g, const Ray& R, float a0, float a1,
                                         int idx[3], float& len) {
     // idx: truncated index per axis.  EXACT maps values outside int32 to
     // INT32_MIN as numpy does on x86-64 (needed only where raw indices are
     // returned); otherwise they saturate, which is equally outside the grid
     // (NaN cannot occur: rays with NaN endpoints fail the gate).
-    if (R.f64) {
+    if (!FAST && R.f64) {
         double A0 = (double)a0, A1 = (double)a1;
         double dA = A1 - A0;
         double m = 0.5 * (A1 + A0);
@@ -397,32 +454,53 @@ __device__ __forceinline__ bool segment(const GridK& g, const Ray& R, float a0, 
 }
 
 // End-trim tests, siddon.py:789-793 (np.isclose default atol=1e-8).
+template <bool FAST = false>
 __device__ __forceinline__ bool trim_close(const Ray& R, float x, float y) {
-    return R.f64 ? isclose_d((double)x, (double)y, 1e-8) : isclose_f(x, y, 1e-8f);
+    return (!FAST && R.f64) ? isclose_d((double)x, (double)y, 1e-8) : isclose_f(x, y, 1e-8f);
 }
 
 // Walk the merged list from the current walker state and call
 // visit(ix, iy, iz, len) for every entry the reference would emit, in
 // order.  Stops once the left end of the next pair exceeds a_stop.
-template <bool P2, bool ISO, bool EXACT = false, class Visit>
+// FAST: every lane's ray satisfies planes_unguarded (float32 list, divisions
+// inside the guard band); see walk_dispatch.
+template <bool P2, bool ISO, bool EXACT = false, bool FAST = false, class Visit>
 __device__ __forceinline__ void walk_entries(const GridK& g, const Ray& R, Walker& W,
                                              float a_stop, Visit&& visit) {
     if (W.consumed == 0) {
         if (W.remaining == 0) return;
-        W.prev = walker_pop<ISO>(g, R, W);
+        W.prev = walker_pop<ISO, FAST>(g, R, W);
         W.consumed = 1;
     }
     while (W.remaining > 0 && !(W.prev > a_stop)) {
-        float b = walker_pop<ISO>(g, R, W);
+        float b = walker_pop<ISO, FAST>(g, R, W);
         bool skip = false;
-        if (W.consumed == 1) skip = trim_close(R, W.prev, b);     // drop A[0]
-        if (W.remaining == 0) skip |= trim_close(R, b, W.prev);   // drop A[-1]
+        if (W.consumed == 1) skip = trim_close<FAST>(R, W.prev, b);     // drop A[0]
+        if (W.remaining == 0) skip |= trim_close<FAST>(R, b, W.prev);   // drop A[-1]
         int idx[3];
         float len;
-        bool keep = segment<P2, EXACT>(g, R, W.prev, b, idx, len);
+        bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
         if (keep && !skip) visit(idx[0], idx[1], idx[2], len);
         W.prev = b;
         W.consumed = 2;
+    }
+}
+
+template <bool B>
+struct Fast {
+    static constexpr bool value = B;
+};
+
+// Call body(Fast<true>{}) when every active lane's ray passes
+// planes_unguarded (the common case: source outside the volume, no
+// axis-parallel ray), else body(Fast<false>{}).  The choice is wave-uniform,
+// and both instantiations produce the same bits.
+template <class Body>
+__device__ __forceinline__ void walk_dispatch(const GridK& g, const Ray& R, Body&& body) {
+    if (__ballot(!planes_unguarded(g, R)) == 0) {
+        body(Fast<true>{});
+    } else {
+        body(Fast<false>{});
     }
 }
 
