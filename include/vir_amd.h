@@ -32,6 +32,8 @@ extern "C" {
 
 #define VIR_ACCUMULATE 1u   /* add into the output instead of overwriting  */
 #define VIR_SEQUENTIAL 2u   /* plain sequential f32 sums in reference order */
+#define VIR_UNGUARDED 4u    /* Siddon FP/BP: vir_siddon_count_guarded gave 0
+                               for these rays and grid */
 
 /* Voxel grid: Grid3d(nPixels, dPixels, origin, float32) of
  * core/core_funcs.py:235-265 plus the snapping epsilon of
@@ -66,6 +68,17 @@ int vir_siddon_fill(const float *src, const float *trg, int64_t nrays,
                     const vir_grid *grid, const int64_t *offsets,
                     int32_t *ravel_idx, int32_t *ix, int32_t *iy, int32_t *iz,
                     float *len, void *stream);
+
+/* Adds to *count (one device int32, initialised by the caller) the number of
+ * rays that meet the grid and need the range-checked plane division or the
+ * float64 alpha list (a zero direction component) while tracing.  When the
+ * count is 0, vir_siddon_fp / vir_siddon_bp calls on the same rays and grid
+ * may pass VIR_UNGUARDED: their kernels are then built without those paths
+ * (same bits, fewer registers).  Passing the flag for other rays gives
+ * results that are not the reference's (memory stays in bounds).  No
+ * reference counterpart: a precondition check for the flag. */
+int vir_siddon_count_guarded(const float *src, const float *trg, int64_t nrays,
+                             const vir_grid *grid, int32_t *count, void *stream);
 
 /* On-the-fly forward projection sino[r] = sum_j vol[idx_j] * len_j over the
  * ray's Siddon entries, with no stored matrix.  Same result as

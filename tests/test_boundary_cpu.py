@@ -49,6 +49,8 @@ def test_invalid_arguments_rejected():
     g = _lib.make_grid((4, 4, 1), (1, 1, 1), (0, 0, 0), 1e-5)
     assert lib.vir_siddon_fp(None, None, 3, g, None, None, 0, None, None) == -1
     assert lib.vir_siddon_bp(None, None, 0, 1, 1, None, g, None, None, 0, None, None) == -1
+    assert lib.vir_siddon_count_guarded(None, None, 1, g, None, None) == -1
+    assert lib.vir_siddon_count_guarded(None, None, 0, g, None, None) == -1  # count is NULL
     assert lib.vir_csr_transpose(None, None, None, 1, 1 << 31, 8, None, None, None, None) == -1
     assert lib.vir_siddon_bp_workspace(g, 0) == -1
 

@@ -160,6 +160,43 @@ def test_fp_bp_against_oracle(kind):
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
 
 
+@pytest.mark.parametrize("kind", ["par2d", "fan2d", "cone3d", "par3d"])
+def test_unguarded_kernels_same_bits(kind):
+    """vir_siddon_count_guarded finds no ray needing the range-checked
+    division on the cone-beam set (source outside every plane, no zero
+    direction component), every ray of the 2-D and parallel 3-D sets needs
+    the float64 list (zero z component); where the count is 0, FP and BP
+    with VIR_UNGUARDED equal the dynamically dispatched kernels bit for bit."""
+    import torch
+    from vir_amd import _lib
+    from vir_amd.device import stream
+    sd = _sd()
+    rng = np.random.default_rng(11)
+    s, t, nP, dP = _geometry(kind, rng)
+    print("case", kind, flush=True)
+    P = sd.SiddonProjector(s, t, nP, dP)
+    if kind == "cone3d":
+        assert P.guarded_rays == 0 and P._ung == _lib.VIR_UNGUARDED
+    else:
+        assert P.guarded_rays > 0 and P._ung == 0
+        return
+    L = _lib.lib()
+    x = torch.from_numpy(rng.uniform(0, 1, P.nvox).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.uniform(0, 1, P.nrays).astype(np.float32)).cuda()
+    outs = []
+    for fl in (0, _lib.VIR_UNGUARDED):
+        f = torch.empty(P.nrays, dtype=torch.float32, device="cuda")
+        b = torch.empty(P.nvox, dtype=torch.float32, device="cuda")
+        _lib.check(L.vir_siddon_fp(_lib.ptr(P.src), _lib.ptr(P.trg), P.nrays, P.grid, _lib.ptr(x),
+                                   _lib.ptr(f), fl, None, stream()), "fp")
+        nviews, nu, nv = P.layout
+        _lib.check(L.vir_siddon_bp(_lib.ptr(P.src), _lib.ptr(P.trg), nviews, nu, nv, _lib.ptr(P.proj),
+                                   P.grid, _lib.ptr(y), _lib.ptr(b), fl, _lib.ptr(P._ws), stream()), "bp")
+        outs.append((f.cpu().numpy(), b.cpu().numpy()))
+    assert np.array_equal(outs[0][0].view(np.int32), outs[1][0].view(np.int32))
+    assert np.array_equal(outs[0][1].view(np.int32), outs[1][1].view(np.int32))
+
+
 @pytest.mark.parametrize("kind", ["par2d", "cone3d"])
 def test_bp_single_rays_exact(kind):
     """Backprojecting a one-hot sinogram reproduces that ray's own entries

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libvir_amd.so")
 VIR_OK = 0
 VIR_ACCUMULATE = 1
 VIR_SEQUENTIAL = 2
+VIR_UNGUARDED = 4
 
 P = ctypes.c_void_p
 I64 = ctypes.c_int64
@@ -82,6 +83,7 @@ SIGNATURES = {
     "vir_siddon_count": (ctypes.c_int, [P, P, I64, G, P, P, P]),
     "vir_siddon_fill": (ctypes.c_int, [P, P, I64, G, P, P, P, P, P, P, P]),
     "vir_siddon_fp": (ctypes.c_int, [P, P, I64, G, P, P, U32, P, P]),
+    "vir_siddon_count_guarded": (ctypes.c_int, [P, P, I64, G, P, P]),
     "vir_siddon_bp_workspace": (I64, [G, I32]),
     "vir_siddon_bp": (ctypes.c_int, [P, P, I32, I32, I32, P, G, P, P, U32, P, P]),
     "vir_csr_fp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),

@@ -82,9 +82,25 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fill(GridK g, const floa
     });
 }
 
+// Rays that meet the grid and need the guarded division or the float64
+// list (planes_unguarded false); one atomic per wave.
+__global__ __launch_bounds__(kRayThreads) void k_siddon_count_guarded(GridK g, const float* __restrict__ src,
+                                                                     const float* __restrict__ trg,
+                                                                     int64_t nrays, int32_t* __restrict__ count) {
+    int64_t r = (int64_t)blockIdx.x * kRayThreads + threadIdx.x;
+    bool need = false;
+    if (r < nrays) {
+        Ray R;
+        ray_setup_core(g, src + 3 * r, trg + 3 * r, R);
+        need = R.hit && !planes_unguarded(g, R);
+    }
+    unsigned long long m = __ballot(need);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int)__popcll(m));
+}
+
 // ------------------------------------------------------------ forward
 
-template <bool P2, bool ISO>
+template <bool P2, bool ISO, bool UNG>
 __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_siddon_fp(GridK g, const float* __restrict__ src,
                                                           const float* __restrict__ trg, int64_t nrays,
                                                           const float* __restrict__ vol,
@@ -105,7 +121,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
         Walker W;
         walker_begin(g, R, W);
         const unsigned nyz = (unsigned)(g.n[1] * g.n[2]);
-        walk_dispatch(g, R, [&](auto fast) {
+        walk_dispatch<UNG>(g, R, [&](auto fast) {
             walk_entries<P2, ISO, false, decltype(fast)::value>(
                 g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
                     bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
@@ -152,7 +168,7 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
 // footprint apart, so the lanes of a wave never update the same voxel in the
 // same colour phase, and consecutive phases of one wave are ordered by the
 // in-order LDS pipeline -- no atomics and no workgroup barriers.
-template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, bool P2, bool ISO>
+template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, bool P2, bool ISO, bool UNG>
 __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WPE))) void k_siddon_bp_wave(GridK g, const float* __restrict__ src,
                                                       const float* __restrict__ trg, int nviews, int nu,
                                                       int nv, const float* __restrict__ proj,
@@ -299,7 +315,7 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     if (miss || alo > ahi) continue;
                     // exact plane slices only matter near the ray's ends
                     if (!window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
-                    walk_dispatch(g, R, [&](auto fast) {
+                    walk_dispatch<UNG>(g, R, [&](auto fast) {
                         constexpr bool F = decltype(fast)::value;
                         Walker W;
                         walker_seek<F>(g, R, W, alo);
@@ -457,9 +473,37 @@ int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_g
     if (!vol || !sino) return fail_arg("vol/sino is NULL");
     int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
     int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
-    VIR_DISPATCH_RAY_KERNEL(k_siddon_fp, g, nb, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g,
-                            src, trg, nrays, vol, sino, acc, err_count);
+    const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
+#define VIR_FP_LAUNCH(P2_, ISO_, UNG_)                                                                       \
+    hipLaunchKernelGGL((k_siddon_fp<P2_, ISO_, UNG_>), dim3((unsigned)nb), dim3(kRayThreads), 0,             \
+                       as_stream(stream), g, src, trg, nrays, vol, sino, acc, err_count)
+    if (ung) {
+        if (p2 && iso) VIR_FP_LAUNCH(true, true, true);
+        else if (p2) VIR_FP_LAUNCH(true, false, true);
+        else if (iso) VIR_FP_LAUNCH(false, true, true);
+        else VIR_FP_LAUNCH(false, false, true);
+    } else {
+        if (p2 && iso) VIR_FP_LAUNCH(true, true, false);
+        else if (p2) VIR_FP_LAUNCH(true, false, false);
+        else if (iso) VIR_FP_LAUNCH(false, true, false);
+        else VIR_FP_LAUNCH(false, false, false);
+    }
+#undef VIR_FP_LAUNCH
     return check_hip(hipGetLastError(), "vir_siddon_fp launch");
+}
+
+int vir_siddon_count_guarded(const float* src, const float* trg, int64_t nrays, const vir_grid* grid,
+                             int32_t* count, void* stream) {
+    GridK g;
+    int rc = make_grid(grid, g);
+    if (rc) return rc;
+    if ((rc = check_rays(src, trg, nrays))) return rc;
+    if (!count) return fail_arg("count is NULL");
+    if (nrays == 0) return VIR_OK;
+    int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
+    hipLaunchKernelGGL(k_siddon_count_guarded, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g, src,
+                       trg, nrays, count);
+    return check_hip(hipGetLastError(), "vir_siddon_count_guarded launch");
 }
 
 int64_t vir_siddon_bp_workspace(const vir_grid* grid, int32_t nviews) {
@@ -495,15 +539,21 @@ int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu
     }
     int nb = p.nbx * p.nby * p.nbz;
     dim3 grd((unsigned)nb, (unsigned)p.nchunks);
-    const bool p2 = all_pow2(g), iso = isotropic(g);
-#define VIR_BP_LAUNCH(SHAPE)                                                                                   \
+    const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
+#define VIR_BP_LAUNCH1(SHAPE, UNG_)                                                                            \
     do {                                                                                                         \
         dim3 blk(64 * (SHAPE##_WX) * (SHAPE##_WY));                                                              \
-        if (p2 && iso) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, true>), grd, blk, 0, st,          \
+        if (p2 && iso) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, true, UNG_>), grd, blk, 0, st,    \
                                           VIR_BP_ARGS);                                                          \
-        else if (p2) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, false>), grd, blk, 0, st,           \
+        else if (p2) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, false, UNG_>), grd, blk, 0, st,     \
                                         VIR_BP_ARGS);                                                            \
-        else hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, false, false>), grd, blk, 0, st, VIR_BP_ARGS);     \
+        else hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, false, false, UNG_>), grd, blk, 0, st,            \
+                                VIR_BP_ARGS);                                                                    \
+    } while (0)
+#define VIR_BP_LAUNCH(SHAPE)                                                                                   \
+    do {                                                                                                         \
+        if (ung) VIR_BP_LAUNCH1(SHAPE, true);                                                                    \
+        else VIR_BP_LAUNCH1(SHAPE, false);                                                                       \
     } while (0)
 #define VIR_BP_ARGS g, src, trg, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
 #define BP3_ARGS 16, 16, 8, 2, 2, 4
@@ -522,6 +572,7 @@ int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu
 #undef BP2_WY
 #undef VIR_BP_ARGS
 #undef VIR_BP_LAUNCH
+#undef VIR_BP_LAUNCH1
     rc = check_hip(hipGetLastError(), "vir_siddon_bp launch");
     if (rc) return rc;
     if (!direct) {

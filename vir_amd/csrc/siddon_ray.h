@@ -492,15 +492,21 @@ struct Fast {
 };
 
 // Call body(Fast<true>{}) when every active lane's ray passes
-// planes_unguarded (the common case: source outside the volume, no
-// axis-parallel ray), else body(Fast<false>{}).  The choice is wave-uniform,
-// and both instantiations produce the same bits.
-template <class Body>
+// planes_unguarded (the common case), else body(Fast<false>{}).  The choice
+// is wave-uniform, and both instantiations produce the same bits.  UNG: the
+// caller has established that every ray passes (vir_siddon_count_guarded),
+// so only the unguarded walk is compiled -- the kernel then needs fewer
+// registers.
+template <bool UNG = false, class Body>
 __device__ __forceinline__ void walk_dispatch(const GridK& g, const Ray& R, Body&& body) {
-    if (__ballot(!planes_unguarded(g, R)) == 0) {
+    if constexpr (UNG) {
         body(Fast<true>{});
     } else {
-        body(Fast<false>{});
+        if (__ballot(!planes_unguarded(g, R)) == 0) {
+            body(Fast<true>{});
+        } else {
+            body(Fast<false>{});
+        }
     }
 }
 

@@ -177,6 +177,15 @@ class SiddonProjector:
                         raise
                     self.proj = None
         self._err = torch.zeros(1, dtype=torch.int32, device=self.src.device)
+        # rays whose tracing needs the range-checked division or a float64
+        # alpha list; when there are none, FP/BP run kernels built without
+        # those paths (same bits, fewer registers)
+        guarded = torch.zeros(1, dtype=torch.int32, device=self.src.device)
+        _lib.check(_lib.lib().vir_siddon_count_guarded(_lib.ptr(self.src), _lib.ptr(self.trg), self.nrays,
+                                                       self.grid, _lib.ptr(guarded), stream()),
+                   "vir_siddon_count_guarded")
+        self.guarded_rays = int(guarded.item())
+        self._ung = _lib.VIR_UNGUARDED if self.guarded_rays == 0 else 0
         ws = 0
         if self.proj is not None:
             ws = int(_lib.lib().vir_siddon_bp_workspace(self.grid, self.layout[0]))
@@ -200,7 +209,7 @@ class SiddonProjector:
             out = torch.empty(self.ray_shape, dtype=torch.float32, device=vol.device)
         if check:
             self._err.zero_()
-        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._ung
         _lib.check(_lib.lib().vir_siddon_fp(_lib.ptr(self.src), _lib.ptr(self.trg), self.nrays,
                                             self.grid, _lib.ptr(vol), _lib.ptr(out), fl,
                                             _lib.ptr(self._err) if check else None, stream()),
@@ -219,6 +228,7 @@ class SiddonProjector:
             out = torch.empty(self.nPixels, dtype=torch.float32, device=sino.device)
         fl = _lib.VIR_ACCUMULATE if accumulate else 0
         if self.proj is not None:
+            fl |= self._ung
             nviews, nu, nv = self.layout
             _lib.check(_lib.lib().vir_siddon_bp(_lib.ptr(self.src), _lib.ptr(self.trg), nviews, nu,
                                                 nv, _lib.ptr(self.proj), self.grid, _lib.ptr(sino),
