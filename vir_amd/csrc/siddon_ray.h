@@ -289,6 +289,7 @@ struct Walker {
     i3v kdir;      // +1 ascending plane index, -1 descending
     f3v h;         // alpha of plane kn, +inf once the stream is exhausted
     int spec;      // 0: alpha_min pending, 1: alpha_max pending, 2: none
+    float sp;      // the pending special value (alpha_min, alpha_max or +inf)
     int remaining; // elements not yet consumed
     float prev;    // last consumed element
     int consumed;  // elements consumed so far, saturating at 2
@@ -310,6 +311,7 @@ __device__ __forceinline__ void walker_begin(const GridK& g, const Ray& R, Walke
         W.h[a] = W.kn[a] != W.kstop[a] ? plane_alpha(g, R, a, W.kn[a]) : VIR_INF;
     }
     W.spec = 0;
+    W.sp = R.amin;
     W.consumed = 0;
     W.prev = 0.0f;
 }
@@ -326,10 +328,14 @@ __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker
     bool c2 = h2 < hm;
     hm = c2 ? h2 : hm;
     int am = c2 ? 2 : (c1 ? 1 : 0);
-    float sv = W.spec == 0 ? R.amin : R.amax;
     W.remaining -= 1;
-    if (W.spec < 2 && !(hm < sv)) {
+    // the special value goes first on ties.  Once both are consumed sp is
+    // +inf; on the FAST path every plane alpha is finite (|q| <= 2^96), so
+    // while elements remain hm < sp.  The general path keeps the count test.
+    if ((FAST || W.spec < 2) && !(hm < W.sp)) {
+        float sv = W.sp;
         W.spec += 1;
+        W.sp = W.spec == 1 ? R.amax : VIR_INF;
         return sv;
     }
     bool s0 = am == 0, s1 = am == 1, s2 = am == 2;
@@ -412,6 +418,7 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
     W.spec = 0;
     if (R.amin <= T) { W.spec = 1; cnt += 1; pv = fmaxf(pv, R.amin); }
     if (W.spec == 1 && R.amax <= T) { W.spec = 2; cnt += 1; pv = fmaxf(pv, R.amax); }
+    W.sp = W.spec == 0 ? R.amin : (W.spec == 1 ? R.amax : VIR_INF);
     W.remaining -= W.spec;
     W.consumed = cnt > 2 ? 2 : cnt;
     W.prev = pv;
@@ -461,28 +468,43 @@ __device__ __forceinline__ bool trim_close(const Ray& R, float x, float y) {
 
 // Walk the merged list from the current walker state and call
 // visit(ix, iy, iz, len) for every entry the reference would emit, in
-// order.  Stops once the left end of the next pair exceeds a_stop.
+// order.  Stops once the left end of the next pair exceeds a_stop.  The
+// first pair (A[0], A[1]) and the last pair are peeled off the loop: only
+// they are subject to the end trims.
 // FAST: every lane's ray satisfies planes_unguarded (float32 list, divisions
 // inside the guard band); see walk_dispatch.
 template <bool P2, bool ISO, bool EXACT = false, bool FAST = false, class Visit>
 __device__ __forceinline__ void walk_entries(const GridK& g, const Ray& R, Walker& W,
                                              float a_stop, Visit&& visit) {
+    int idx[3];
+    float len;
     if (W.consumed == 0) {
         if (W.remaining == 0) return;
         W.prev = walker_pop<ISO, FAST>(g, R, W);
         W.consumed = 1;
     }
-    while (W.remaining > 0 && !(W.prev > a_stop)) {
+    if (W.consumed == 1) {
+        if (W.remaining == 0 || W.prev > a_stop) return;
         float b = walker_pop<ISO, FAST>(g, R, W);
-        bool skip = false;
-        if (W.consumed == 1) skip = trim_close<FAST>(R, W.prev, b);     // drop A[0]
-        if (W.remaining == 0) skip |= trim_close<FAST>(R, b, W.prev);   // drop A[-1]
-        int idx[3];
-        float len;
+        bool skip = trim_close<FAST>(R, W.prev, b);                   // drop A[0]
+        if (W.remaining == 0) skip |= trim_close<FAST>(R, b, W.prev);  // drop A[-1]
         bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
         if (keep && !skip) visit(idx[0], idx[1], idx[2], len);
         W.prev = b;
         W.consumed = 2;
+    }
+    while (W.remaining > 1 && !(W.prev > a_stop)) {
+        float b = walker_pop<ISO, FAST>(g, R, W);
+        bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
+        if (keep) visit(idx[0], idx[1], idx[2], len);
+        W.prev = b;
+    }
+    if (W.remaining == 1 && !(W.prev > a_stop)) {
+        float b = walker_pop<ISO, FAST>(g, R, W);
+        bool skip = trim_close<FAST>(R, b, W.prev);                    // drop A[-1]
+        bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
+        if (keep && !skip) visit(idx[0], idx[1], idx[2], len);
+        W.prev = b;
     }
 }
 
