@@ -216,8 +216,11 @@ typedef struct vir_sf_geom {
 } vir_sf_geom;
 
 /* sino = out_scale * sum over pixels of img * amplitude * overlap/du
- * (* overlap_v/dv); one thread per bin.  The reference scales by d_pix
- * (pd.py:234, 350, 485). */
+ * (* overlap_v/dv).  The reference scales by d_pix (pd.py:234, 350, 485).
+ * Parallel 3-D with rows no taller than a voxel runs a tiled kernel that
+ * stages each orthogonal line's pixels in LDS (same bits as the one-thread-
+ * per-bin gather, which VIR_SF_PER_BIN selects). */
+#define VIR_SF_PER_BIN 8u
 int vir_sf_fp(const vir_sf_geom *geom, const float *ang, const float *cosv,
               const float *sinv, const float *img, float *sino, float out_scale,
               uint32_t flags, void *stream);

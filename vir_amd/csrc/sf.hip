@@ -109,9 +109,10 @@ __device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, const SFView& V, int i
 
 // Rows (FP: slabs) a slab (row) of the parallel 3-D geometry meets, with
 // their weights overlap/dv: independent of the pixel and the view, so
-// computed once per thread.  Up to kSlabs entries; more (rows much taller
-// than voxels) fall back to the general loop.
-constexpr int kSlabs = 4;
+// computed once per thread.  Up to kSlabs entries -- enough for rows no
+// taller than a voxel, rounding included; more (taller rows) fall back to
+// the general loop.
+constexpr int kSlabs = 5;
 struct SlabWeights {
     int first, count;
     float w[kSlabs];
@@ -119,6 +120,32 @@ struct SlabWeights {
 
 __device__ __forceinline__ float ovl(float a0, float a1, float b0, float b1) {
     return fminf(a1, b1) - fmaxf(a0, b0);
+}
+
+// Slabs that detector row iv of the parallel 3-D geometry meets, with their
+// weights overlap/dv (zero-padded to kSlabs).  Returns false when the row
+// meets more than kSlabs slabs (rows taller than a voxel).
+__device__ __forceinline__ bool par3_row_slabs(const SFGeo& G, int iv, SlabWeights& SW) {
+    const float vl = sf_det(G.nv, G.dv, G.sv, iv), vr = sf_det(G.nv, G.dv, G.sv, iv + 1);
+    const float inv_d = 1.0f / G.d, inv_dv = 1.0f / G.dv;
+    float z0f = fminf(vl, vr) * inv_d + 0.5f * (float)G.nz;
+    float z1f = fmaxf(vl, vr) * inv_d + 0.5f * (float)G.nz;
+    int z0 = (int)fmaxf(floorf(z0f) - 1.0f, 0.0f);
+    int z1 = (int)fminf(floorf(z1f) + 1.0f, (float)(G.nz - 1));
+    SW.first = z0;
+    SW.count = z1 - z0 + 1;
+    if (z1 - z0 + 1 > kSlabs) {
+#pragma unroll
+        for (int k = 0; k < kSlabs; ++k) SW.w[k] = 0.0f;
+        return false;
+    }
+#pragma unroll
+    for (int k = 0; k < kSlabs; ++k) {
+        const int z = z0 + k;
+        float ov = z <= z1 ? ovl(sf_bnd(G.nz, G.d, z), sf_bnd(G.nz, G.d, z + 1), vl, vr) : 0.0f;
+        SW.w[k] = ov > 0.0f ? ov * inv_dv : 0.0f;
+    }
+    return true;
 }
 
 // centre projection along the driving axis, inverted: the drive coordinate
@@ -162,24 +189,7 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
     SW.first = 0;
     SW.count = 0;
     bool slab_general = false;
-    if (MODE == SF_PAR3) {
-        float z0f = fminf(vl, vr) * inv_d + 0.5f * (float)G.nz;
-        float z1f = fmaxf(vl, vr) * inv_d + 0.5f * (float)G.nz;
-        int z0 = (int)fmaxf(floorf(z0f) - 1.0f, 0.0f);
-        int z1 = (int)fminf(floorf(z1f) + 1.0f, (float)(G.nz - 1));
-        SW.first = z0;
-        if (z1 - z0 + 1 > kSlabs) {
-            slab_general = true;
-        } else {
-#pragma unroll
-            for (int k = 0; k < kSlabs; ++k) {
-                const int z = z0 + k;
-                float ov = z <= z1 ? ovl(sf_bnd(G.nz, G.d, z), sf_bnd(G.nz, G.d, z + 1), vl, vr) : 0.0f;
-                SW.w[k] = ov > 0.0f ? ov * inv_dv : 0.0f;
-            }
-            SW.count = z1 - z0 + 1;
-        }
-    }
+    if (MODE == SF_PAR3) slab_general = !par3_row_slabs(G, iv, SW);
     double acc = 0.0;
     for (int io = 0; io < no; ++io) {
         const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
@@ -221,6 +231,165 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
     float res = (float)acc * G.out_scale;
     if (accumulate) res += sino[t];
     sino[t] = res;
+}
+
+// ------------------------------------------ tiled parallel 3-D forward
+//
+// One workgroup computes a tile of kTU bins x kTV detector rows of one view
+// of one batch item.  For every orthogonal line the pixels whose footprints
+// can meet the tile (the union of the per-bin ranges of k_sf_fp) are staged
+// in LDS with all the slabs the tile's rows meet (z-contiguous, coalesced),
+// then
+//   P[k][m]  = sum_q w_q(m) * vox[k][first(m) + q]   (slab weights of row m)
+//   acc[u][m] += wu(k, u) * P[k][m]                   (footprint weights)
+// with P written over the staged row it is computed from.  Weights, pixel
+// order and arithmetic are those of k_sf_fp, so both give the same bits.
+constexpr int kTU = 64;          // detector bins per tile
+constexpr int kTV = 64;          // detector rows per tile (one per lane)
+constexpr int kKMax = 128;       // pixels of one line staged at once
+constexpr int kZMax = 72;        // slabs staged: kTV rows of height <= d meet <= kTV + kSlabs
+constexpr int kZPad = kZMax + 1; // row stride (odd: spreads LDS banks)
+constexpr int kWMax = 12;        // pixels whose footprint meets one bin (checked on the host)
+
+__global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
+                                                        const float* __restrict__ cs,
+                                                        const float* __restrict__ sn,
+                                                        const float* __restrict__ img, float* __restrict__ sino,
+                                                        int accumulate, int nut, int nmt) {
+    __shared__ float tile[kKMax * kZPad];
+    __shared__ int wk[kTU * kWMax];
+    __shared__ float ww[kTU * kWMax];
+    __shared__ int wcnt[kTU];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int blk = blockIdx.x;
+    const int mt = blk % nmt;
+    blk /= nmt;
+    const int ut = blk % nut;
+    blk /= nut;
+    const int ia = blk % G.na;
+    const int b = blk / G.na;
+    const int u_lo = ut * kTU, u_n = min(kTU, G.nu - u_lo);
+    const int m_lo = mt * kTV, m_n = min(kTV, G.nv - m_lo);
+
+    const SFView V = sf_view(G, cs[ia], sn[ia], ang[ia]);
+    const float c = V.c, s = V.s;
+    const bool xd = fabsf(s) >= fabsf(c);
+    const int nd = xd ? G.nx : G.ny, no = xd ? G.ny : G.nx;
+    const float inv_d = 1.0f / G.d, inv_du = 1.0f / G.du;
+    const float* vol = img + (int64_t)b * G.nx * G.ny * G.nz;
+
+    // this lane's detector row and its slabs; the tile's slab range
+    const bool mrow = lane < m_n;
+    SlabWeights SW;
+    par3_row_slabs(G, m_lo + (mrow ? lane : 0), SW);
+    SlabWeights S0, S1;
+    par3_row_slabs(G, m_lo, S0);
+    par3_row_slabs(G, m_lo + m_n - 1, S1);
+    const int zt0 = S0.first;
+    const int nzt = min(min(S1.first + kSlabs - 1, G.nz - 1) - zt0 + 1, kZMax);
+
+    // the tile's bin edges (footprint margin included)
+    const float ut0 = sf_det(G.nu, G.du, G.su, u_lo) - G.wmax;
+    const float ut1 = sf_det(G.nu, G.du, G.su, u_lo + u_n) + G.wmax;
+
+    double acc[kTU / 4];
+#pragma unroll
+    for (int i = 0; i < kTU / 4; ++i) acc[i] = 0.0;
+
+    for (int io = 0; io < no; ++io) {
+        const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
+        const float x0 = sf_invert<SF_PAR3>(G, xd, c, s, ut0, oc);
+        const float x1 = sf_invert<SF_PAR3>(G, xd, c, s, ut1, oc);
+        const float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
+        const float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
+        if (!(k0f < (float)nd + 1.0f) || !(k1f > -2.0f)) continue;
+        const int k0t = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
+        const int k1t = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
+        for (int kc = k0t; kc <= k1t; kc += kKMax) {
+            const int nk = min(kKMax, k1t - kc + 1);
+            __syncthreads();  // the previous chunk's reads are done
+            // stage the chunk's voxels: tile[kk][z - zt0]
+            for (int kk = wave; kk < nk; kk += 4) {
+                const int k = kc + kk;
+                const int ix = xd ? k : io, iy = xd ? io : k;
+                const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz + zt0;
+                for (int z = lane; z < nzt; z += 64) tile[kk * kZPad + z] = col[z];
+            }
+            // footprint weights of the tile's bins, pixels of this chunk only
+            if (wave == 0) {
+                int cnt = 0;
+                if (lane < u_n) {
+                    const int iu = u_lo + lane;
+                    const float ul = sf_det(G.nu, G.du, G.su, iu), ur = sf_det(G.nu, G.du, G.su, iu + 1);
+                    const float y0 = sf_invert<SF_PAR3>(G, xd, c, s, ul - G.wmax, oc);
+                    const float y1 = sf_invert<SF_PAR3>(G, xd, c, s, ur + G.wmax, oc);
+                    const float b0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
+                    const float b1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
+                    if (b0f < (float)nd + 1.0f && b1f > -2.0f) {
+                        const int kb0 = max((int)fmaxf(floorf(b0f) - 1.0f, 0.0f), kc);
+                        const int kb1 = min((int)fminf(floorf(b1f) + 2.0f, (float)(nd - 1)), kc + nk - 1);
+                        for (int k = kb0; k <= kb1; ++k) {
+                            const int ix = xd ? k : io, iy = xd ? io : k;
+                            const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
+                            const float ou = ovl(f.u0, f.u1, ul, ur);
+                            if (!(ou > 0.0f)) continue;
+                            if (cnt < kWMax) {
+                                wk[lane * kWMax + cnt] = k - kc;
+                                ww[lane * kWMax + cnt] = (ou * inv_du) * f.amp;
+                            }
+                            ++cnt;
+                        }
+                    }
+                }
+                wcnt[lane] = min(cnt, kWMax);
+            }
+            __syncthreads();
+            // P[kk][m] over the staged row kk (each row belongs to one wave)
+            if (mrow) {
+                for (int kk = wave; kk < nk; kk += 4) {
+                    float sv_ = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < kSlabs; ++q) {
+                        const int z = SW.first + q - zt0;
+                        if (SW.w[q] > 0.0f && z < nzt) sv_ += tile[kk * kZPad + z] * SW.w[q];
+                    }
+                    tile[kk * kZPad + lane] = sv_;
+                }
+            }
+            __syncthreads();
+            if (mrow) {
+#pragma unroll
+                for (int i = 0; i < kTU / 4; ++i) {
+                    const int tu = wave * (kTU / 4) + i;
+                    if (tu < u_n) {
+                        const int cnt = wcnt[tu];
+                        for (int j = 0; j < cnt; ++j)
+                            acc[i] += (double)(ww[tu * kWMax + j] * tile[wk[tu * kWMax + j] * kZPad + lane]);
+                    }
+                }
+            }
+        }
+    }
+    if (!mrow) return;
+    const int64_t base = (((int64_t)b * G.na + ia) * G.nu) * G.nv + m_lo + lane;
+#pragma unroll
+    for (int i = 0; i < kTU / 4; ++i) {
+        const int tu = wave * (kTU / 4) + i;
+        if (tu >= u_n) continue;
+        const int64_t o = base + (int64_t)(u_lo + tu) * G.nv;
+        float res = (float)acc[i] * G.out_scale;
+        if (accumulate) res += sino[o];
+        sino[o] = res;
+    }
+}
+
+// True when the tiled parallel 3-D forward kernel applies: detector rows no
+// taller than a voxel (each row meets at most kSlabs slabs, a tile's rows
+// at most kZMax) and at most kWMax - 1 pixels per bin along a line.
+static bool sf_par3_tiled_ok(const SFGeo& G) {
+    if (!(G.dv <= G.d)) return false;
+    const double per_bin = ((double)G.du + 2.0 * (double)G.wmax) / (0.70710678 * (double)G.d) + 5.0;
+    return per_bin <= (double)(kWMax - 1);
 }
 
 template <int MODE>
@@ -377,7 +546,16 @@ int vir_sf_fp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
         case SF_PAR2: hipLaunchKernelGGL(k_sf_fp<SF_PAR2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
         case SF_FAN2: hipLaunchKernelGGL(k_sf_fp<SF_FAN2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
         case SF_CONE3: hipLaunchKernelGGL(k_sf_fp<SF_CONE3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
-        default: hipLaunchKernelGGL(k_sf_fp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
+        default:
+            if (sf_par3_tiled_ok(G) && !(flags & VIR_SF_PER_BIN)) {
+                const int nut = (G.nu + kTU - 1) / kTU, nmt = (G.nv + kTV - 1) / kTV;
+                const int64_t nblk = (int64_t)G.nb * G.na * nut * nmt;
+                hipLaunchKernelGGL(k_sf_fp_par3_tile, dim3((unsigned)nblk), dim3(256), 0, st, G, ang, cosv, sinv,
+                                   img, sino, acc, nut, nmt);
+            } else {
+                hipLaunchKernelGGL(k_sf_fp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc);
+            }
+            break;
     }
     return check_hip(hipGetLastError(), "vir_sf_fp launch");
 }
