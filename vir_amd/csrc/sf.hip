@@ -236,31 +236,35 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
 // ------------------------------------------ tiled parallel 3-D forward
 //
 // One workgroup computes a tile of kTU bins x kTV detector rows of one view
-// of one batch item.  For every orthogonal line the pixels whose footprints
-// can meet the tile (the union of the per-bin ranges of k_sf_fp) are staged
-// in LDS with all the slabs the tile's rows meet (z-contiguous, coalesced),
-// then
-//   P[k][m]  = sum_q w_q(m) * vox[k][first(m) + q]   (slab weights of row m)
-//   acc[u][m] += wu(k, u) * P[k][m]                   (footprint weights)
-// with P written over the staged row it is computed from.  Weights, pixel
-// order and arithmetic are those of k_sf_fp, so both give the same bits.
-constexpr int kTU = 64;          // detector bins per tile
+// of one batch item; lane m of every wave owns detector row m_lo + m.  For
+// every orthogonal line, over the pixels whose footprints can meet the tile
+// (the union of the per-bin ranges of k_sf_fp):
+//   P[k][m] = sum over the row's slabs q of w_q(m) * vox[k][first(m) + q]
+// read straight from the volume (lane m reads z = first(m) + q: consecutive
+// lanes, consecutive z) into LDS, while each wave lists, for its 16 bins,
+// the pixels whose footprints meet the bin and their weights; then
+//   acc[u][m] += wu(k, u) * P[k][m]
+// Weights, pixel order and arithmetic are those of k_sf_fp, so both give
+// the same bits.
+constexpr int kTU = 64;          // detector bins per tile (16 per wave)
 constexpr int kTV = 64;          // detector rows per tile (one per lane)
-constexpr int kKMax = 128;       // pixels of one line staged at once
-constexpr int kZMax = 72;        // slabs staged: kTV rows of height <= d meet <= kTV + kSlabs
-constexpr int kZPad = kZMax + 1; // row stride (odd: spreads LDS banks)
+constexpr int kKMax = 128;       // pixels of one line handled at once
 constexpr int kWMax = 12;        // pixels whose footprint meets one bin (checked on the host)
+constexpr int kWSlot = kWMax;    // footprint list stride per bin (LDS: 4 workgroups per CU)
+constexpr int kWPer = 3;         // candidate pixels per lane: 4 lanes x 3 cover kWMax
 
 __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
                                                         const float* __restrict__ cs,
                                                         const float* __restrict__ sn,
                                                         const float* __restrict__ img, float* __restrict__ sino,
                                                         int accumulate, int nut, int nmt) {
-    __shared__ float tile[kKMax * kZPad];
-    __shared__ int wk[kTU * kWMax];
-    __shared__ float ww[kTU * kWMax];
+    __shared__ float P[kKMax * kTV];
+    // footprint list per bin: (LDS offset (k - kc) * kTV of the j-th pixel,
+    // its weight overlap/du * amplitude as bits), read as one 64-bit word
+    __shared__ int2 wlist[kTU * kWSlot];
     __shared__ int wcnt[kTU];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar
     int blk = blockIdx.x;
     const int mt = blk % nmt;
     blk /= nmt;
@@ -278,15 +282,38 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
     const float inv_d = 1.0f / G.d, inv_du = 1.0f / G.du;
     const float* vol = img + (int64_t)b * G.nx * G.ny * G.nz;
 
-    // this lane's detector row and its slabs; the tile's slab range
+    // this lane's detector row: its slabs with non-zero weight, in slab
+    // order (the k_sf_fp sum skips the zero weights)
     const bool mrow = lane < m_n;
     SlabWeights SW;
     par3_row_slabs(G, m_lo + (mrow ? lane : 0), SW);
-    SlabWeights S0, S1;
-    par3_row_slabs(G, m_lo, S0);
-    par3_row_slabs(G, m_lo + m_n - 1, S1);
-    const int zt0 = S0.first;
-    const int nzt = min(min(S1.first + kSlabs - 1, G.nz - 1) - zt0 + 1, kZMax);
+    int nzs = 0, sz[kSlabs];
+    float swt[kSlabs];
+#pragma unroll
+    for (int q = 0; q < kSlabs; ++q) {
+        sz[q] = 0;
+        swt[q] = 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < kSlabs; ++q) {
+        if (SW.w[q] > 0.0f) {
+#pragma unroll
+            for (int r = 0; r < kSlabs; ++r) {  // slot nzs, without dynamic register indexing
+                if (r == nzs) {
+                    sz[r] = SW.first + q;
+                    swt[r] = SW.w[q];
+                }
+            }
+            ++nzs;
+        }
+    }
+    if (!mrow) nzs = 0;
+
+    // footprint-list lanes: bin tu_w (this wave's 16 bins), candidate group grp
+    const int tu_w = wave * (kTU / 4) + (lane & 15), grp = lane >> 4;
+    const int iu_w = u_lo + tu_w;
+    const bool ubin = tu_w < u_n;
+    const float ul = sf_det(G.nu, G.du, G.su, iu_w), ur = sf_det(G.nu, G.du, G.su, iu_w + 1);
 
     // the tile's bin edges (footprint margin included)
     const float ut0 = sf_det(G.nu, G.du, G.su, u_lo) - G.wmax;
@@ -305,66 +332,88 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
         if (!(k0f < (float)nd + 1.0f) || !(k1f > -2.0f)) continue;
         const int k0t = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
         const int k1t = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
+        // this lane's bin: its candidate pixel range (as k_sf_fp)
+        int kb0 = 0, kb1 = -1;
+        if (ubin) {
+            const float y0 = sf_invert<SF_PAR3>(G, xd, c, s, ul - G.wmax, oc);
+            const float y1 = sf_invert<SF_PAR3>(G, xd, c, s, ur + G.wmax, oc);
+            const float b0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
+            const float b1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
+            if (b0f < (float)nd + 1.0f && b1f > -2.0f) {
+                kb0 = (int)fmaxf(floorf(b0f) - 1.0f, 0.0f);
+                kb1 = (int)fminf(floorf(b1f) + 2.0f, (float)(nd - 1));
+            }
+        }
         for (int kc = k0t; kc <= k1t; kc += kKMax) {
             const int nk = min(kKMax, k1t - kc + 1);
             __syncthreads();  // the previous chunk's reads are done
-            // stage the chunk's voxels: tile[kk][z - zt0]
+            // P rows of this wave's pixels, straight from the volume
             for (int kk = wave; kk < nk; kk += 4) {
                 const int k = kc + kk;
                 const int ix = xd ? k : io, iy = xd ? io : k;
-                const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz + zt0;
-                for (int z = lane; z < nzt; z += 64) tile[kk * kZPad + z] = col[z];
-            }
-            // footprint weights of the tile's bins, pixels of this chunk only
-            if (wave == 0) {
-                int cnt = 0;
-                if (lane < u_n) {
-                    const int iu = u_lo + lane;
-                    const float ul = sf_det(G.nu, G.du, G.su, iu), ur = sf_det(G.nu, G.du, G.su, iu + 1);
-                    const float y0 = sf_invert<SF_PAR3>(G, xd, c, s, ul - G.wmax, oc);
-                    const float y1 = sf_invert<SF_PAR3>(G, xd, c, s, ur + G.wmax, oc);
-                    const float b0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
-                    const float b1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
-                    if (b0f < (float)nd + 1.0f && b1f > -2.0f) {
-                        const int kb0 = max((int)fmaxf(floorf(b0f) - 1.0f, 0.0f), kc);
-                        const int kb1 = min((int)fminf(floorf(b1f) + 2.0f, (float)(nd - 1)), kc + nk - 1);
-                        for (int k = kb0; k <= kb1; ++k) {
-                            const int ix = xd ? k : io, iy = xd ? io : k;
-                            const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
-                            const float ou = ovl(f.u0, f.u1, ul, ur);
-                            if (!(ou > 0.0f)) continue;
-                            if (cnt < kWMax) {
-                                wk[lane * kWMax + cnt] = k - kc;
-                                ww[lane * kWMax + cnt] = (ou * inv_du) * f.amp;
-                            }
-                            ++cnt;
-                        }
-                    }
-                }
-                wcnt[lane] = min(cnt, kWMax);
-            }
-            __syncthreads();
-            // P[kk][m] over the staged row kk (each row belongs to one wave)
-            if (mrow) {
-                for (int kk = wave; kk < nk; kk += 4) {
-                    float sv_ = 0.0f;
+                const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
+                float xs[kSlabs];
 #pragma unroll
-                    for (int q = 0; q < kSlabs; ++q) {
-                        const int z = SW.first + q - zt0;
-                        if (SW.w[q] > 0.0f && z < nzt) sv_ += tile[kk * kZPad + z] * SW.w[q];
+                for (int q = 0; q < kSlabs; ++q)
+                    if (q < nzs) xs[q] = col[sz[q]];
+                float sv_ = 0.0f;
+#pragma unroll
+                for (int q = 0; q < kSlabs; ++q)
+                    if (q < nzs) sv_ += xs[q] * swt[q];
+                P[kk * kTV + lane] = sv_;
+            }
+            // footprint lists of this wave's bins over the chunk, in pixel
+            // order: lane group grp tests candidates ka + 3 grp .. ka + 3 grp + 2
+            {
+                const int ka = max(kb0, kc), kz = min(kb1, kc + nk - 1);
+                float wv[kWPer];
+                int nzc = 0;
+#pragma unroll
+                for (int j = 0; j < kWPer; ++j) {
+                    const int k = ka + grp * kWPer + j;
+                    float w = 0.0f;
+                    if (k <= kz) {
+                        const int ix = xd ? k : io, iy = xd ? io : k;
+                        const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
+                        const float ou = ovl(f.u0, f.u1, ul, ur);
+                        if (ou > 0.0f) w = (ou * inv_du) * f.amp;
                     }
-                    tile[kk * kZPad + lane] = sv_;
+                    wv[j] = w;
+                    nzc += (k <= kz && w != 0.0f) ? 1 : 0;
                 }
+                // exclusive prefix of the group counts of this bin (lanes l, l+16, l+32, l+48)
+                const int c1 = __shfl_up(nzc, 16), c2 = __shfl_up(nzc, 32), c3 = __shfl_up(nzc, 48);
+                int pos = (grp >= 1 ? c1 : 0) + (grp >= 2 ? c2 : 0) + (grp >= 3 ? c3 : 0);
+                const int total = __shfl(pos + nzc, (lane & 15) + 48);
+#pragma unroll
+                for (int j = 0; j < kWPer; ++j) {
+                    const int k = ka + grp * kWPer + j;
+                    if (k <= kz && wv[j] != 0.0f) {
+                        wlist[tu_w * kWSlot + pos] = make_int2((k - kc) * kTV, __float_as_int(wv[j]));
+                        ++pos;
+                    }
+                }
+                if (grp == 0) wcnt[tu_w] = ubin ? total : 0;
             }
             __syncthreads();
             if (mrow) {
 #pragma unroll
                 for (int i = 0; i < kTU / 4; ++i) {
                     const int tu = wave * (kTU / 4) + i;
-                    if (tu < u_n) {
-                        const int cnt = wcnt[tu];
-                        for (int j = 0; j < cnt; ++j)
-                            acc[i] += (double)(ww[tu * kWMax + j] * tile[wk[tu * kWMax + j] * kZPad + lane]);
+                    const int cnt = wcnt[tu];  // wave-uniform
+                    const int2* wl = wlist + tu * kWSlot;
+                    int j = 0;
+                    // offsets (k - kc) * kTV with 0 <= k - kc < nk <= kKMax
+                    for (; j + 1 < cnt; j += 2) {
+                        const int2 e0 = wl[j], e1 = wl[j + 1];
+                        const float p0 = P[e0.x + lane];
+                        const float p1 = P[e1.x + lane];
+                        acc[i] += (double)(__int_as_float(e0.y) * p0);
+                        acc[i] += (double)(__int_as_float(e1.y) * p1);
+                    }
+                    if (j < cnt) {
+                        const int2 e0 = wl[j];
+                        acc[i] += (double)(__int_as_float(e0.y) * P[e0.x + lane]);
                     }
                 }
             }
@@ -384,8 +433,8 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
 }
 
 // True when the tiled parallel 3-D forward kernel applies: detector rows no
-// taller than a voxel (each row meets at most kSlabs slabs, a tile's rows
-// at most kZMax) and at most kWMax - 1 pixels per bin along a line.
+// taller than a voxel (each row meets at most kSlabs slabs) and at most
+// kWMax - 1 pixels per bin along a line.
 static bool sf_par3_tiled_ok(const SFGeo& G) {
     if (!(G.dv <= G.d)) return false;
     const double per_bin = ((double)G.du + 2.0 * (double)G.wmax) / (0.70710678 * (double)G.d) + 5.0;
