@@ -225,7 +225,10 @@ int vir_sf_fp(const vir_sf_geom *geom, const float *ang, const float *cosv,
               const float *sinv, const float *img, float *sino, float out_scale,
               uint32_t flags, void *stream);
 
-/* Exact transpose of vir_sf_fp, one thread per pixel/voxel.  The reference's
+/* Exact transpose of vir_sf_fp, one thread per pixel/voxel (parallel 3-D
+ * with rows at least half a voxel tall: one wave per pixel column with the
+ * footprints of 64 views computed at once, same bits; VIR_SF_PER_BIN selects
+ * the per-voxel kernel).  The reference's
  * backprojectors use out_scale = du/(na*d_pix) (2-D, pd.py:231-232, 347-348)
  * and du*dv/(na*d_pix) (cone, pd.py:482-483). */
 int vir_sf_bp(const vir_sf_geom *geom, const float *ang, const float *cosv,

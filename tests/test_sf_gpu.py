@@ -111,7 +111,9 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     """The LDS-tiled parallel 3-D forward kernel gives the same bits as the
     one-thread-per-bin gather (VIR_SF_PER_BIN): two bin tiles, two row
     tiles, lines whose pixel band exceeds one staged chunk (du = 2 d), a
-    45-degree view, a batch of two."""
+    45-degree view, a batch of two.  Likewise the column backprojector
+    against the one-thread-per-voxel kernel (two slab tiles, 76 views: two
+    view groups)."""
     import torch
     from vir_amd import _lib
     from vir_amd.device import stream
@@ -126,6 +128,18 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     _lib.check(_lib.lib().vir_sf_fp(P.g, _lib.ptr(P.ang), _lib.ptr(P.cos), _lib.ptr(P.sin), _lib.ptr(x),
                                     _lib.ptr(gather), P.fp_scale, _lib.VIR_SF_PER_BIN, stream()), "fp")
     a, b = tiled.cpu().numpy(), gather.cpu().numpy()
+    assert np.abs(b).max() > 0
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    # backprojection: column kernel vs per-voxel kernel
+    shape2, nu2, nv2 = (24, 20, 70), 40, 60
+    ang2 = np.concatenate([np.linspace(0, 2 * np.pi, 75, endpoint=False), [np.pi / 4]])
+    P2 = SFProjector(shape2, ang2, nu2, "par3d", nv=nv2, du=0.9, dv=0.6, su=0.2, sv=0.1, d_pix=1.0, batch=2)
+    y = torch.from_numpy(rng.normal(size=P2.sino_shape).astype(np.float32)).cuda()
+    col = P2.bp(y)
+    vox = torch.empty_like(col)
+    _lib.check(_lib.lib().vir_sf_bp(P2.g, _lib.ptr(P2.ang), _lib.ptr(P2.cos), _lib.ptr(P2.sin), _lib.ptr(y),
+                                    _lib.ptr(vox), P2.fp_scale, _lib.VIR_SF_PER_BIN, stream()), "bp")
+    a, b = col.cpu().numpy(), vox.cpu().numpy()
     assert np.abs(b).max() > 0
     assert np.array_equal(a.view(np.int32), b.view(np.int32))
 

@@ -148,6 +148,31 @@ __device__ __forceinline__ bool par3_row_slabs(const SFGeo& G, int iv, SlabWeigh
     return true;
 }
 
+// Detector rows that slab iz of the parallel 3-D geometry meets, with their
+// weights overlap/dv (zero-padded to kSlabs).  Returns false when the slab
+// meets more than kSlabs rows (rows much shorter than a voxel).
+__device__ __forceinline__ bool par3_slab_rows(const SFGeo& G, int iz, SlabWeights& SW) {
+    const float zl = sf_bnd(G.nz, G.d, iz), zr = sf_bnd(G.nz, G.d, iz + 1);
+    const float v0 = sf_det(G.nv, G.dv, G.sv, 0), inv_dv = 1.0f / G.dv;
+    const float lo = fminf(zl, zr), hi = fmaxf(zl, zr);
+    int m0 = (int)fmaxf(floorf((lo - v0) * inv_dv) - 1.0f, 0.0f);
+    int m1 = (int)fminf(floorf((hi - v0) * inv_dv) + 1.0f, (float)(G.nv - 1));
+    SW.first = m0;
+    SW.count = m1 - m0 + 1;
+    if (m1 - m0 + 1 > kSlabs) {
+#pragma unroll
+        for (int k = 0; k < kSlabs; ++k) SW.w[k] = 0.0f;
+        return false;
+    }
+#pragma unroll
+    for (int k = 0; k < kSlabs; ++k) {
+        const int m = m0 + k;
+        float ov = m <= m1 ? ovl(zl, zr, sf_det(G.nv, G.dv, G.sv, m), sf_det(G.nv, G.dv, G.sv, m + 1)) : 0.0f;
+        SW.w[k] = ov > 0.0f ? ov * inv_dv : 0.0f;
+    }
+    return true;
+}
+
 // centre projection along the driving axis, inverted: the drive coordinate
 // whose pixel centre projects to u on orthogonal line coordinate oc
 template <int MODE>
@@ -432,6 +457,131 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
     }
 }
 
+// ------------------------------------------ parallel 3-D backward by columns
+//
+// One wave per (batch item, pixel, 64 slabs): lane z accumulates voxel z of
+// the column over all views.  The footprint of the pixel -- bins with
+// positive overlap and their weights -- is computed for 64 views at once
+// (lane = view) and broadcast with readlane in the view loop, which then only
+// reads the sinogram rows each slab meets (consecutive lanes, consecutive
+// rows) and accumulates.  Weights, bin and view order and arithmetic are
+// those of k_sf_bp, so both give the same bits.
+constexpr int kJMax = 8;  // bins a pixel footprint meets (checked on the host)
+
+__global__ __launch_bounds__(256) void k_sf_bp_par3_col(SFGeo G, const float* __restrict__ ang,
+                                                       const float* __restrict__ cs,
+                                                       const float* __restrict__ sn,
+                                                       const float* __restrict__ sino, float* __restrict__ img,
+                                                       int accumulate, int nzt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wid >= (int64_t)G.nb * G.nx * G.ny * nzt) return;  // whole wave
+    const int zt = (int)(wid % nzt);
+    int64_t r = wid / nzt;
+    const int iy = (int)(r % G.ny);
+    r /= G.ny;
+    const int ix = (int)(r % G.nx);
+    const int b = (int)(r / G.nx);
+    const int iz = zt * 64 + lane;
+    const bool zin = iz < G.nz;
+
+    // this lane's slab: rows with non-zero weight, in row order (k_sf_bp skips
+    // the zero weights, and the whole view when the slab meets no row)
+    SlabWeights SW;
+    par3_slab_rows(G, zin ? iz : 0, SW);
+    int nzr = 0, rq[kSlabs];
+    float rw[kSlabs];
+#pragma unroll
+    for (int q = 0; q < kSlabs; ++q) {
+        rq[q] = 0;
+        rw[q] = 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < kSlabs; ++q) {
+        if (SW.w[q] > 0.0f) {
+#pragma unroll
+            for (int s = 0; s < kSlabs; ++s) {  // slot nzr, without dynamic register indexing
+                if (s == nzr) {
+                    rq[s] = SW.first + q;
+                    rw[s] = SW.w[q];
+                }
+            }
+            ++nzr;
+        }
+    }
+    if (!zin || SW.count <= 0) nzr = 0;
+
+    const float u0 = sf_det(G.nu, G.du, G.su, 0);
+    const float inv_du = 1.0f / G.du;
+    const float* sb = sino + (int64_t)b * G.na * G.nu * G.nv;
+    double acc = 0.0;
+    for (int va = 0; va < G.na; va += 64) {
+        // lane = view va + lane: bins of this pixel's footprint with positive overlap
+        int cnt = 0, jl[kJMax];
+        float wl[kJMax];
+#pragma unroll
+        for (int s = 0; s < kJMax; ++s) {
+            jl[s] = 0;
+            wl[s] = 0.0f;
+        }
+        const int ial = va + lane;
+        if (ial < G.na) {
+            const SFView V = sf_view(G, cs[ial], sn[ial], ang[ial]);
+            const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
+            if (f.u1 > u0 - G.du && (f.u0 - u0) * inv_du < (float)G.nu + 1.0f) {
+                const int j0 = (int)fmaxf(floorf((f.u0 - u0) * inv_du) - 1.0f, 0.0f);
+                const int j1 = (int)fminf(floorf((f.u1 - u0) * inv_du) + 1.0f, (float)(G.nu - 1));
+#pragma unroll
+                for (int t = 0; t < kJMax; ++t) {
+                    const int j = j0 + t;
+                    if (j > j1) break;
+                    const float ou = ovl(f.u0, f.u1, sf_det(G.nu, G.du, G.su, j), sf_det(G.nu, G.du, G.su, j + 1));
+                    if (!(ou > 0.0f)) continue;
+                    const float wu = (ou * inv_du) * f.amp;
+#pragma unroll
+                    for (int s = 0; s < kJMax; ++s) {
+                        if (s == cnt) {
+                            jl[s] = j;
+                            wl[s] = wu;
+                        }
+                    }
+                    ++cnt;
+                }
+            }
+        }
+        const int nvw = min(64, G.na - va);
+        for (int v = 0; v < nvw; ++v) {
+            const int c = __builtin_amdgcn_readlane(cnt, v);
+            const float* plane = sb + (int64_t)(va + v) * G.nu * G.nv;
+#pragma unroll
+            for (int s = 0; s < kJMax; ++s) {
+                if (s >= c) break;
+                const int j = __builtin_amdgcn_readlane(jl[s], v);
+                const float wu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wl[s]), v));
+                const float* row = plane + (int64_t)j * G.nv;
+                float sv_ = 0.0f;
+#pragma unroll
+                for (int q = 0; q < kSlabs; ++q)
+                    if (q < nzr) sv_ += row[rq[q]] * rw[q];
+                acc += (double)(wu * sv_);
+            }
+        }
+    }
+    if (!zin) return;
+    const int64_t o = (((int64_t)b * G.nx + ix) * G.ny + iy) * G.nz + iz;
+    float res = (float)acc * G.out_scale;
+    if (accumulate) res += img[o];
+    img[o] = res;
+}
+
+// True when the column backprojector applies: a slab meets at most kSlabs
+// detector rows (rows at least half a voxel tall) and a pixel footprint at
+// most kJMax bins.
+static bool sf_par3_bp_col_ok(const SFGeo& G) {
+    if (!((double)G.d / (double)G.dv <= 1.99)) return false;
+    return 1.4143 * (double)G.d / (double)G.du + 4.0 <= (double)kJMax;
+}
+
 // True when the tiled parallel 3-D forward kernel applies: detector rows no
 // taller than a voxel (each row meets at most kSlabs slabs) and at most
 // kWMax - 1 pixels per bin along a line.
@@ -465,23 +615,7 @@ __global__ __launch_bounds__(256) void k_sf_bp(SFGeo G, const float* __restrict_
     SW.first = 0;
     SW.count = 0;
     bool row_general = false;
-    if (MODE == SF_PAR3) {
-        const float lo = fminf(zl, zr), hi = fmaxf(zl, zr);
-        int m0 = (int)fmaxf(floorf((lo - v0) * inv_dv) - 1.0f, 0.0f);
-        int m1 = (int)fminf(floorf((hi - v0) * inv_dv) + 1.0f, (float)(nv - 1));
-        SW.first = m0;
-        if (m1 - m0 + 1 > kSlabs) {
-            row_general = true;
-        } else {
-#pragma unroll
-            for (int k = 0; k < kSlabs; ++k) {
-                const int m = m0 + k;
-                float ov = m <= m1 ? ovl(zl, zr, sf_det(nv, G.dv, G.sv, m), sf_det(nv, G.dv, G.sv, m + 1)) : 0.0f;
-                SW.w[k] = ov > 0.0f ? ov * inv_dv : 0.0f;
-            }
-            SW.count = m1 - m0 + 1;
-        }
-    }
+    if (MODE == SF_PAR3) row_general = !par3_slab_rows(G, iz, SW);
     double acc = 0.0;
     for (int ia = 0; ia < G.na; ++ia) {
         const SFView V = sf_view(G, cs[ia], sn[ia], ang[ia]);
@@ -624,7 +758,16 @@ int vir_sf_bp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
         case SF_PAR2: hipLaunchKernelGGL(k_sf_bp<SF_PAR2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
         case SF_FAN2: hipLaunchKernelGGL(k_sf_bp<SF_FAN2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
         case SF_CONE3: hipLaunchKernelGGL(k_sf_bp<SF_CONE3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
-        default: hipLaunchKernelGGL(k_sf_bp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
+        default:
+            if (sf_par3_bp_col_ok(G) && !(flags & VIR_SF_PER_BIN)) {
+                const int nzt = (G.nz + 63) / 64;
+                const int64_t nw = (int64_t)G.nb * G.nx * G.ny * nzt;
+                hipLaunchKernelGGL(k_sf_bp_par3_col, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, st, G, ang, cosv,
+                                   sinv, sino, img, acc, nzt);
+            } else {
+                hipLaunchKernelGGL(k_sf_bp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc);
+            }
+            break;
     }
     return check_hip(hipGetLastError(), "vir_sf_bp launch");
 }
