@@ -44,7 +44,7 @@ def test_total_variation_value_and_gradient():
     x64 = torch.from_numpy(x).double().requires_grad_(True)
     ref = _tv64(x64, eps)
     ref.backward()
-    assert abs(float(v.detach()) - float(ref)) <= 1e-6 * abs(float(ref))
+    assert abs(float(v.detach()) - float(ref.detach())) <= 1e-6 * abs(float(ref.detach()))
     g, gr = xt.grad.cpu().double(), x64.grad
     assert float((g - gr).norm() / gr.norm()) <= 1e-5
     # 2-D images: TV over the last two axes only
