@@ -34,6 +34,8 @@ extern "C" {
 #define VIR_SEQUENTIAL 2u   /* plain sequential f32 sums in reference order */
 #define VIR_UNGUARDED 4u    /* Siddon FP/BP: vir_siddon_count_guarded gave 0
                                for these rays and grid */
+#define VIR_PER_OUTPUT 8u   /* DD 2-D / SF: the one-thread-per-output kernels
+                               instead of the tiled ones (same bits) */
 
 /* Voxel grid: Grid3d(nPixels, dPixels, origin, float32) of
  * core/core_funcs.py:235-265 plus the snapping epsilon of
@@ -219,15 +221,14 @@ typedef struct vir_sf_geom {
  * (* overlap_v/dv).  The reference scales by d_pix (pd.py:234, 350, 485).
  * Parallel 3-D with rows no taller than a voxel runs a tiled kernel that
  * stages each orthogonal line's pixels in LDS (same bits as the one-thread-
- * per-bin gather, which VIR_SF_PER_BIN selects). */
-#define VIR_SF_PER_BIN 8u
+ * per-bin gather, which VIR_PER_OUTPUT selects). */
 int vir_sf_fp(const vir_sf_geom *geom, const float *ang, const float *cosv,
               const float *sinv, const float *img, float *sino, float out_scale,
               uint32_t flags, void *stream);
 
 /* Exact transpose of vir_sf_fp, one thread per pixel/voxel (parallel 3-D
  * with rows at least half a voxel tall: one wave per pixel column with the
- * footprints of 64 views computed at once, same bits; VIR_SF_PER_BIN selects
+ * footprints of 64 views computed at once, same bits; VIR_PER_OUTPUT selects
  * the per-voxel kernel).  The reference's
  * backprojectors use out_scale = du/(na*d_pix) (2-D, pd.py:231-232, 347-348)
  * and du*dv/(na*d_pix) (cone, pd.py:482-483). */

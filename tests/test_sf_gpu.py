@@ -109,7 +109,7 @@ def test_par3d_batched_matches_oracle_per_volume():
 
 def test_par3d_tiled_forward_equals_per_bin_gather():
     """The LDS-tiled parallel 3-D forward kernel gives the same bits as the
-    one-thread-per-bin gather (VIR_SF_PER_BIN): two bin tiles, two row
+    one-thread-per-bin gather (VIR_PER_OUTPUT): two bin tiles, two row
     tiles, lines whose pixel band exceeds one staged chunk (du = 2 d), a
     45-degree view, a batch of two.  Likewise the column backprojector
     against the one-thread-per-voxel kernel (two slab tiles, 76 views: two
@@ -126,7 +126,7 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     tiled = P.fp(x)
     gather = torch.empty_like(tiled)
     _lib.check(_lib.lib().vir_sf_fp(P.g, _lib.ptr(P.ang), _lib.ptr(P.cos), _lib.ptr(P.sin), _lib.ptr(x),
-                                    _lib.ptr(gather), P.fp_scale, _lib.VIR_SF_PER_BIN, stream()), "fp")
+                                    _lib.ptr(gather), P.fp_scale, _lib.VIR_PER_OUTPUT, stream()), "fp")
     a, b = tiled.cpu().numpy(), gather.cpu().numpy()
     assert np.abs(b).max() > 0
     assert np.array_equal(a.view(np.int32), b.view(np.int32))
@@ -138,7 +138,7 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     col = P2.bp(y)
     vox = torch.empty_like(col)
     _lib.check(_lib.lib().vir_sf_bp(P2.g, _lib.ptr(P2.ang), _lib.ptr(P2.cos), _lib.ptr(P2.sin), _lib.ptr(y),
-                                    _lib.ptr(vox), P2.fp_scale, _lib.VIR_SF_PER_BIN, stream()), "bp")
+                                    _lib.ptr(vox), P2.fp_scale, _lib.VIR_PER_OUTPUT, stream()), "bp")
     a, b = col.cpu().numpy(), vox.cpu().numpy()
     assert np.abs(b).max() > 0
     assert np.array_equal(a.view(np.int32), b.view(np.int32))

@@ -15,7 +15,7 @@ VIR_OK = 0
 VIR_ACCUMULATE = 1
 VIR_SEQUENTIAL = 2
 VIR_UNGUARDED = 4
-VIR_SF_PER_BIN = 8
+VIR_PER_OUTPUT = 8
 
 P = ctypes.c_void_p
 I64 = ctypes.c_int64

@@ -14,6 +14,7 @@
 // inverting the projection; the backward kernel runs one thread per pixel
 // (voxel) and visits the bins its shadow meets.  Both evaluate every weight
 // through the same functions, so they are exact transposes.
+#include <algorithm>
 #include <cmath>
 
 #include "vir_common.h"
@@ -138,6 +139,116 @@ __global__ __launch_bounds__(256) void k_dd2d_fp(DDGeo G, const float* __restric
             pb = pn;
         }
     }
+    float r = (float)acc * G.out_scale;
+    if (accumulate) r += sino[t];
+    sino[t] = r;
+}
+
+// Tiled forward: one workgroup per (view, 256 bins).  For every orthogonal
+// line the projected boundaries of the pixels that can meet the tile's bins
+// (the union of the per-bin ranges of k_dd2d_fp, widened by one) are
+// evaluated once into LDS with the pixel values, and each bin merges its own
+// pixels from there -- on the equiangular detector that is one atan2 per
+// boundary instead of one per (bin, boundary).  The per-line ray scale and
+// the tile's pixel band are computed once per workgroup, and two lines share
+// each pair of barriers.  Same functions, pixel order and arithmetic as
+// k_dd2d_fp: the same bits.
+constexpr int kDDKMax = 256;    // pixels of one line staged at once (per line of a pair)
+constexpr int kDDLines = 2048;  // lines whose scale and band are tabulated per workgroup
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __restrict__ cs,
+                                                     const float* __restrict__ sn, const float* __restrict__ img,
+                                                     float* __restrict__ sino, int accumulate, int nut) {
+    __shared__ float bnd[2][kDDKMax + 1];  // projected boundary kc + e of the two lines
+    __shared__ float val[2][kDDKMax];      // pixel kc + e
+    __shared__ float lsc[kDDLines];        // ray scale of line io
+    __shared__ int2 lband[kDDLines];       // the tile's pixel band of line io (empty: y < x)
+    const int ut = blockIdx.x % nut, ia = blockIdx.x / nut;
+    const int u_lo = ut * 256, u_n = min(256, G.nu - u_lo);
+    const int iu = u_lo + (int)threadIdx.x;
+    const bool ubin = (int)threadIdx.x < u_n;
+    const DDView V = dd_view(cs[ia], sn[ia]);
+    const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
+    const float ul = det_bnd(G.nu, G.du, G.su, iu), ur = det_bnd(G.nu, G.du, G.su, iu + 1);
+    const float tl = dd_slope<MODE>(G, ul), tr = dd_slope<MODE>(G, ur);
+    const float tt0 = dd_slope<MODE>(G, det_bnd(G.nu, G.du, G.su, u_lo));
+    const float tt1 = dd_slope<MODE>(G, det_bnd(G.nu, G.du, G.su, u_lo + u_n));
+    const float inv_d = 1.0f / G.d;
+    // per-line table (host guarantees no <= kDDLines)
+    for (int io = threadIdx.x; io < no; io += 256) {
+        const float oc = cell_cen(no, G.d, io);
+        const float y0 = dd_invert<MODE>(G, V, tt0, oc), y1 = dd_invert<MODE>(G, V, tt1, oc);
+        const float t0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd;
+        const float t1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd;
+        int2 bandv = make_int2(0, -1);
+        if (t0f < (float)nd + 3.0f && t1f > -3.0f) {
+            bandv.x = (int)fmaxf(floorf(t0f) - 2.0f, 0.0f);
+            bandv.y = (int)fminf(floorf(t1f) + 2.0f, (float)(nd - 1));
+        }
+        lband[io] = bandv;
+        lsc[io] = dd_scale<MODE>(G, V, oc);
+    }
+    __syncthreads();
+    // stage boundaries and values of pixels kc .. kc + nk - 1 of line io into slot h
+    auto stage = [&](int h, int io, int kc, int nk) {
+        const float oc = cell_cen(no, G.d, io);
+        for (int e = threadIdx.x; e <= nk; e += 256) {
+            bnd[h][e] = dd_bound<MODE>(G, V, cell_bnd(nd, G.d, kc + e), oc);
+            if (e < nk) {
+                const int k = kc + e;
+                const int ix = V.xd ? k : io, iy = V.xd ? io : k;
+                val[h][e] = img[(int64_t)ix * G.ny + iy];
+            }
+        }
+    };
+    double acc = 0.0;
+    // this bin's pixels of line io within the staged chunk, as k_dd2d_fp
+    auto merge = [&](int h, int io, int kc, int nk) {
+        const float oc = cell_cen(no, G.d, io);
+        const float x0 = dd_invert<MODE>(G, V, tl, oc), x1 = dd_invert<MODE>(G, V, tr, oc);
+        const float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd;
+        const float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd;
+        if (!(k0f < (float)nd + 2.0f) || !(k1f > -2.0f)) return;
+        const int k0 = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
+        const int k1 = (int)fminf(floorf(k1f) + 1.0f, (float)(nd - 1));
+        const float sc = lsc[io];
+        const int ka = max(k0, kc), kz = min(k1, kc + nk - 1);
+        for (int k = ka; k <= kz; ++k) {
+            const float ov = overlap(bnd[h][k - kc], bnd[h][k - kc + 1], ul, ur);
+            if (ov > 0.0f) acc += (double)(val[h][k - kc] * (ov * sc));
+        }
+    };
+    for (int io0 = 0; io0 < no; io0 += 2) {
+        const int2 b0 = lband[io0];
+        const int2 b1 = io0 + 1 < no ? lband[io0 + 1] : make_int2(0, -1);
+        const int n0 = b0.y - b0.x + 1, n1 = b1.y - b1.x + 1;
+        if (n0 <= 0 && n1 <= 0) continue;
+        if (n0 <= kDDKMax && n1 <= kDDKMax) {
+            // both lines in one chunk each: one pair of barriers for two lines
+            __syncthreads();
+            if (n0 > 0) stage(0, io0, b0.x, n0);
+            if (n1 > 0) stage(1, io0 + 1, b1.x, n1);
+            __syncthreads();
+            if (ubin && n0 > 0) merge(0, io0, b0.x, n0);
+            if (ubin && n1 > 0) merge(1, io0 + 1, b1.x, n1);
+            continue;
+        }
+        // a long band: each line on its own, chunk by chunk
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int2 bb = h ? b1 : b0;
+            for (int kc = bb.x; kc <= bb.y; kc += kDDKMax) {
+                const int nk = min(kDDKMax, bb.y - kc + 1);
+                __syncthreads();
+                stage(0, io0 + h, kc, nk);
+                __syncthreads();
+                if (ubin) merge(0, io0 + h, kc, nk);
+            }
+        }
+    }
+    if (!ubin) return;
+    const int64_t t = (int64_t)ia * G.nu + iu;
     float r = (float)acc * G.out_scale;
     if (accumulate) r += sino[t];
     sino[t] = r;
@@ -317,6 +428,16 @@ int vir_dd2d_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, c
     dim3 grd((unsigned)((n + 255) / 256));
     int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
     hipStream_t st = as_stream(stream);
+    if (!(flags & VIR_PER_OUTPUT) && std::max(G.nx, G.ny) <= kDDLines) {
+        const int nut = (G.nu + 255) / 256;
+        dim3 tg((unsigned)((int64_t)G.na * nut));
+        if (geom->mode == DD_PAR)
+            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_PAR>, tg, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, nut);
+        else if (geom->mode == DD_FAN_FLAT)
+            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_FLAT>, tg, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, nut);
+        else hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_ARC>, tg, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, nut);
+        return check_hip(hipGetLastError(), "vir_dd2d_fp launch");
+    }
     if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_fp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
     else if (geom->mode == DD_FAN_FLAT)
         hipLaunchKernelGGL(k_dd2d_fp<DD_FAN_FLAT>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);

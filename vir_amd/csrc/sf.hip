@@ -730,7 +730,7 @@ int vir_sf_fp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
         case SF_FAN2: hipLaunchKernelGGL(k_sf_fp<SF_FAN2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
         case SF_CONE3: hipLaunchKernelGGL(k_sf_fp<SF_CONE3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
         default:
-            if (sf_par3_tiled_ok(G) && !(flags & VIR_SF_PER_BIN)) {
+            if (sf_par3_tiled_ok(G) && !(flags & VIR_PER_OUTPUT)) {
                 const int nut = (G.nu + kTU - 1) / kTU, nmt = (G.nv + kTV - 1) / kTV;
                 const int64_t nblk = (int64_t)G.nb * G.na * nut * nmt;
                 hipLaunchKernelGGL(k_sf_fp_par3_tile, dim3((unsigned)nblk), dim3(256), 0, st, G, ang, cosv, sinv,
@@ -759,7 +759,7 @@ int vir_sf_bp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
         case SF_FAN2: hipLaunchKernelGGL(k_sf_bp<SF_FAN2>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
         case SF_CONE3: hipLaunchKernelGGL(k_sf_bp<SF_CONE3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, sino, img, acc); break;
         default:
-            if (sf_par3_bp_col_ok(G) && !(flags & VIR_SF_PER_BIN)) {
+            if (sf_par3_bp_col_ok(G) && !(flags & VIR_PER_OUTPUT)) {
                 const int nzt = (G.nz + 63) / 64;
                 const int64_t nw = (int64_t)G.nb * G.nx * G.ny * nzt;
                 hipLaunchKernelGGL(k_sf_bp_par3_col, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, st, G, ang, cosv,
