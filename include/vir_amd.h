@@ -236,6 +236,28 @@ int vir_sf_bp(const vir_sf_geom *geom, const float *ang, const float *cosv,
               const float *sinv, const float *sino, float *img, float out_scale,
               uint32_t flags, void *stream);
 
+/* ------------------------------------------ filtered backprojection */
+
+/* out[r, i] = sum_j weight[j] sino[r, j] taps[|i - j|] for nrows rows of nu
+ * bins (nu <= 4096; weight may be NULL).  With taps[k] = irfft(2 H)[k] of a
+ * windowed ramp H this equals filtration.filter_sino (filtration.py:267-329):
+ * rFFT of the row zero-padded to z >= 2 nu, times 2 H, inverse, first nu
+ * samples.  Not in place. */
+int vir_fbp_filter(const float *sino, int64_t nrows, int32_t nu,
+                   const float *weight, const float *taps, float *out,
+                   void *stream);
+
+/* img = scale * sum over views of the weighted, linearly interpolated
+ * filtered projection q (na, nu) at each pixel centre, in the DD projector's
+ * geometry (vir_dd_geom; mode 0 parallel: detector coordinate, weight 1;
+ * mode 1 flat fan: virtual-detector coordinate DSO p/(DSO - o), weight
+ * (DSO/(DSO - o))^2; mode 2 equiangular fan: fan angle, weight 1/L^2).  The
+ * pixel-driven backprojection of the reference's fan FBP
+ * (processing/reconstruction.py:7-65, processing/proj2.py:580-614). */
+int vir_fbp_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
+               const float *q, float *img, float scale, uint32_t flags,
+               void *stream);
+
 /* ------------------------------------------ ray-driven (AW / Joseph) */
 
 /* Geometry of the ray-driven projectors (sys_mat/rd.py).  mode: 0 parallel

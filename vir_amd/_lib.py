@@ -84,6 +84,8 @@ SIGNATURES = {
     "vir_siddon_count": (ctypes.c_int, [P, P, I64, G, P, P, P]),
     "vir_siddon_fill": (ctypes.c_int, [P, P, I64, G, P, P, P, P, P, P, P]),
     "vir_siddon_fp": (ctypes.c_int, [P, P, I64, G, P, P, U32, P, P]),
+    "vir_fbp_filter": (ctypes.c_int, [P, I64, I32, P, P, P, P]),
+    "vir_fbp_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_siddon_count_guarded": (ctypes.c_int, [P, P, I64, G, P, P]),
     "vir_siddon_bp_workspace": (I64, [G, I32]),
     "vir_siddon_bp": (ctypes.c_int, [P, P, I32, I32, I32, P, G, P, P, U32, P, P]),
