@@ -64,7 +64,7 @@ def test_least_squares():
     v = least_squares(at, torch.from_numpy(b).cuda())
     (3.0 * v).backward()
     d = a.astype(np.float64) - b
-    assert abs(float(v) - 0.5 * (d * d).sum()) <= 1e-9 * (d * d).sum()
+    assert abs(float(v.detach()) - 0.5 * (d * d).sum()) <= 1e-9 * (d * d).sum()
     np.testing.assert_allclose(at.grad.cpu().numpy(), 3.0 * (a - b), rtol=1e-6)
 
 
