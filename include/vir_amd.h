@@ -72,8 +72,9 @@ int vir_siddon_fill(const float *src, const float *trg, int64_t nrays,
                     float *len, void *stream);
 
 /* Adds to *count (one device int32, initialised by the caller) the number of
- * rays that meet the grid and need the range-checked plane division or the
- * float64 alpha list (a zero direction component) while tracing.  When the
+ * rays whose grid entry/exit division needs the range check, or that meet
+ * the grid and need the range-checked plane division or the float64 alpha
+ * list (a zero direction component) while tracing.  When the
  * count is 0, vir_siddon_fp / vir_siddon_bp calls on the same rays and grid
  * may pass VIR_UNGUARDED: their kernels are then built without those paths
  * (same bits, fewer registers).  Passing the flag for other rays gives

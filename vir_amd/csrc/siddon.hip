@@ -82,8 +82,10 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fill(GridK g, const floa
     });
 }
 
-// Rays that meet the grid and need the guarded division or the float64
-// list (planes_unguarded false); one atomic per wave.
+// Rays whose entry/exit divisions need div_rn's range check
+// (entry_unguarded false), and rays that meet the grid and need it for a plane
+// division or take the float64 list (planes_unguarded false); one atomic per
+// wave.
 __global__ __launch_bounds__(kRayThreads) void k_siddon_count_guarded(GridK g, const float* __restrict__ src,
                                                                      const float* __restrict__ trg,
                                                                      int64_t nrays, int32_t* __restrict__ count) {
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_count_guarded(GridK g, c
     if (r < nrays) {
         Ray R;
         ray_setup_core(g, src + 3 * r, trg + 3 * r, R);
-        need = R.hit && !planes_unguarded(g, R);
+        need = !entry_unguarded(g, R) || (R.hit && !planes_unguarded(g, R));
     }
     unsigned long long m = __ballot(need);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int)__popcll(m));
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
     int64_t r = blk * kRayThreads + threadIdx.x;
     if (r >= nrays) return;
     Ray R;
-    ray_setup(g, src + 3 * r, trg + 3 * r, R);
+    ray_setup<UNG>(g, src + 3 * r, trg + 3 * r, R);
     // float64 accumulation of the float32 products (the accumulator of
     // projection/proj_numba.py:17), one-deep software pipeline: the gather
     // issued for entry j is consumed while entry j+1 is being traced
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     float wgt = sino[ray];
                     if (wgt == 0.0f) continue;
                     Ray R;
-                    ray_setup_core(g, src + 3 * ray, trg + 3 * ray, R);
+                    ray_setup_core<UNG>(g, src + 3 * ray, trg + 3 * ray, R);
                     if (!R.hit) continue;
                     // alpha interval of the ray inside the guard box
                     float alo = -VIR_INF, ahi = VIR_INF;

@@ -133,6 +133,9 @@ struct Ray {
 // Per-ray setup without the plane slices: endpoints, direction, length,
 // entry/exit alphas and the gate (siddon.py:705-736, 759-760).  The slices are
 // left at the full plane range [0, n+1).
+// FAST: the caller has established entry_unguarded for this ray, so the six
+// entry/exit divisions return the same bits without div_rn's range check.
+template <bool FAST = false>
 __device__ __forceinline__ void ray_setup_core(const GridK& g, const float* __restrict__ src,
                                                const float* __restrict__ trg, Ray& R) {
 #pragma unroll
@@ -155,8 +158,8 @@ __device__ __forceinline__ void ray_setup_core(const GridK& g, const float* __re
         R.f64 |= z;
         float den = z ? g.eps_f : R.dST[a];
         float iden = z ? g.inv_eps : R.y[a];
-        float al0 = div_rn(g.p0[a] - R.s[a], den, iden);
-        float alN = div_rn(g.pN[a] - R.s[a], den, iden);
+        float al0 = div_rn_t<FAST>(g.p0[a] - R.s[a], den, iden);
+        float alN = div_rn_t<FAST>(g.pN[a] - R.s[a], den, iden);
         float mn = 0.0f, mx = 1.0f;
         if (al0 < alN) { mn = al0; mx = alN; }
         else if (al0 > alN) { mn = alN; mx = al0; }
@@ -199,10 +202,27 @@ __device__ __forceinline__ void ray_setup_slices(const GridK& g, Ray& R) {
 }
 
 // Full per-ray setup, siddon.py:705-751 and the gate at :759-760.
+template <bool FAST = false>
 __device__ __forceinline__ void ray_setup(const GridK& g, const float* __restrict__ src,
                                           const float* __restrict__ trg, Ray& R) {
-    ray_setup_core(g, src, trg, R);
+    ray_setup_core<FAST>(g, src, trg, R);
     ray_setup_slices(g, R);
+}
+
+// True when the six entry/exit divisions of ray_setup_core, (p0 - s) / den
+// and (pN - s) / den per axis, pass div_rn's own range check, so
+// ray_setup_core<true> computes the same bits (hit or not).
+__device__ __forceinline__ bool entry_unguarded(const GridK& g, const Ray& R) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float iden = R.dST[a] == 0.0f ? g.inv_eps : R.y[a];
+        float n0 = g.p0[a] - R.s[a], n1 = g.pN[a] - R.s[a];
+        float q0 = fabsf(n0 * iden), q1 = fabsf(n1 * iden);
+        ok = ok && ((fabsf(n0) >= 0x1p-96f && q0 >= 0x1p-100f && q0 <= 0x1p100f) || n0 == 0.0f);
+        ok = ok && ((fabsf(n1) >= 0x1p-96f && q1 >= 0x1p-100f && q1 <= 0x1p100f) || n1 == 0.0f);
+    }
+    return ok;
 }
 
 // True when every plane alpha of the ray, RN((plane - s) / dST) over the
@@ -386,14 +406,20 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
             // first k in [lo, hi) with alpha(k) > T
             k = fk < -1.0e9f ? lo : (fk > 1.0e9f ? hi : (int)fk + 1);
             k = k < lo ? lo : (k > hi ? hi : k);
-            while (k > lo && plane_alpha<FAST>(g, R, a, k - 1) > T) --k;
+            // am1: alpha of plane k - 1 (the last one taken) while k > lo
+            float am1 = k > lo ? plane_alpha<FAST>(g, R, a, k - 1) : -VIR_INF;
+            while (k > lo && am1 > T) {
+                --k;
+                am1 = k > lo ? plane_alpha<FAST>(g, R, a, k - 1) : -VIR_INF;
+            }
             hk = k < hi ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             while (k < hi && hk <= T) {
+                am1 = hk;
                 ++k;
                 hk = k < hi ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             }
             int taken = k - lo;
-            if (taken > 0) pv = fmaxf(pv, plane_alpha<FAST>(g, R, a, k - 1));
+            if (taken > 0) pv = fmaxf(pv, am1);
             cnt += taken;
             W.kn[a] = k; W.kstop[a] = hi; W.kdir[a] = 1;
             W.remaining += hi - k;
@@ -401,14 +427,20 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
             // stream runs hi-1 .. lo; next = largest k in [lo, hi) with alpha(k) > T
             k = fk < -1.0e9f ? lo - 1 : (fk > 1.0e9f ? hi - 1 : (int)fk);
             k = k < lo - 1 ? lo - 1 : (k > hi - 1 ? hi - 1 : k);
-            while (k < hi - 1 && plane_alpha<FAST>(g, R, a, k + 1) > T) ++k;
+            // ap1: alpha of plane k + 1 (the last one taken) while k < hi - 1
+            float ap1 = k < hi - 1 ? plane_alpha<FAST>(g, R, a, k + 1) : -VIR_INF;
+            while (k < hi - 1 && ap1 > T) {
+                ++k;
+                ap1 = k < hi - 1 ? plane_alpha<FAST>(g, R, a, k + 1) : -VIR_INF;
+            }
             hk = k >= lo ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             while (k >= lo && hk <= T) {
+                ap1 = hk;
                 --k;
                 hk = k >= lo ? plane_alpha<FAST>(g, R, a, k) : VIR_INF;
             }
             int taken = (hi - 1) - k;
-            if (taken > 0) pv = fmaxf(pv, plane_alpha<FAST>(g, R, a, k + 1));
+            if (taken > 0) pv = fmaxf(pv, ap1);
             cnt += taken;
             W.kn[a] = k; W.kstop[a] = lo - 1; W.kdir[a] = -1;
             W.remaining += k - (lo - 1);
