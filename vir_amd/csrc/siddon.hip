@@ -115,29 +115,34 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
     ray_setup<UNG>(g, src + 3 * r, trg + 3 * r, R);
     // float64 accumulation of the float32 products (the accumulator of
     // projection/proj_numba.py:17), one-deep software pipeline: the gather
-    // issued for entry j is consumed while entry j+1 is being traced
+    // issued for entry j lands in pv while entry j+1 is being traced, and is
+    // consumed before the gather for j+1 overwrites it.  Every pair issues a
+    // gather (of voxel 0 for dropped and out-of-grid pairs, weighted 0), so
+    // the body has no branch around the load.
     double sum = 0.0;
     int bad = 0;
     float pv = 0.0f, pl = 0.0f;
+    bool pin = false;
     if (R.hit) {
         Walker W;
         walker_begin(g, R, W);
         const unsigned nyz = (unsigned)(g.n[1] * g.n[2]);
         walk_dispatch<UNG>(g, R, [&](auto fast) {
-            walk_entries<P2, ISO, false, decltype(fast)::value>(
-                g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len) {
+            walk_entries<P2, ISO, false, decltype(fast)::value, true>(
+                g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len, bool keep) {
                     bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
                               (unsigned)iz < (unsigned)g.n[2];
-                    bad += in ? 0 : 1;
+                    bad += (keep && !in) ? 1 : 0;
+                    bool use = keep && in;
                     unsigned vi = (unsigned)ix * nyz + (unsigned)iy * (unsigned)g.n[2] + (unsigned)iz;
-                    float v = in ? vol[vi] : 0.0f;
-                    sum += (double)(pv * pl);
-                    pv = v;
-                    pl = in ? len : 0.0f;
+                    sum += (double)((pin ? pv : 0.0f) * pl);
+                    pv = vol[use ? vi : 0u];
+                    pin = use;
+                    pl = use ? len : 0.0f;
                 });
         });
     }
-    sum += (double)(pv * pl);
+    sum += (double)((pin ? pv : 0.0f) * pl);
     float res = (float)sum;
     if (accumulate) res += sino[r];
     sino[r] = res;

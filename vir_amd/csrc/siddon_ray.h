@@ -87,6 +87,13 @@ __device__ __forceinline__ float plane_at(int n, float d, float o, int k) {
     return t + o;
 }
 
+// plane_at from lin = 2 k - n (a float holding that integer exactly).
+__device__ __forceinline__ float plane_lin(float d, float o, float lin) {
+    float t = lin * d;
+    t = t * 0.5f;
+    return t + o;
+}
+
 __device__ __forceinline__ float plane_coord(const GridK& g, int a, int k) {
     return plane_at(g.n[a], g.d[a], g.o[a], k);
 }
@@ -303,11 +310,17 @@ __device__ __forceinline__ T sel3(int a, T v0, T v1, T v2) {
 // (siddon.py:803-809), so the emitted entries do not depend on tie order.
 // Exhausted streams hold +inf: every plane alpha in a stream lies close to
 // [alpha_min, alpha_max] within [0, 1], so +inf never collides with a value.
+// Plane indices are held as the float lin = 2 k - n of plane_at (exact
+// integers: n <= 2^22 per axis), so advancing a stream is one add and the
+// plane coordinate needs no conversion.  Per-axis state is kept in scalar
+// members (x/y/z suffixes 0/1/2): a pick among the three axes then stays two
+// selects on the comparison masks, where a vector element pick would be
+// turned into an index and compares.
 struct Walker {
-    i3v kn;        // next plane index in stream order
-    i3v kstop;     // one past the last plane in stream order
-    i3v kdir;      // +1 ascending plane index, -1 descending
-    f3v h;         // alpha of plane kn, +inf once the stream is exhausted
+    float ln0, ln1, ln2;        // lin of the next plane in stream order
+    float lstop0, lstop1, lstop2;  // lin of one past the last plane in stream order
+    float lstep0, lstep1, lstep2;  // +2 ascending plane index, -2 descending
+    float h0, h1, h2;           // alpha of the next plane, +inf once exhausted
     int spec;      // 0: alpha_min pending, 1: alpha_max pending, 2: none
     float sp;      // the pending special value (alpha_min, alpha_max or +inf)
     int remaining; // elements not yet consumed
@@ -315,21 +328,41 @@ struct Walker {
     int consumed;  // elements consumed so far, saturating at 2
 };
 
-__device__ __forceinline__ int stream_len(const Walker& W, int a) {
-    return W.kdir[a] > 0 ? W.kstop[a] - W.kn[a] : W.kn[a] - W.kstop[a];
+// Per-axis stream state while the setup loops run (vector elements with
+// constant indices once unrolled), stored into the walker's scalars at the end.
+struct StreamInit {
+    f3v ln, lstop, lstep, h;
+};
+
+__device__ __forceinline__ void walker_store(Walker& W, const StreamInit& S) {
+    W.ln0 = S.ln[0]; W.ln1 = S.ln[1]; W.ln2 = S.ln[2];
+    W.lstop0 = S.lstop[0]; W.lstop1 = S.lstop[1]; W.lstop2 = S.lstop[2];
+    W.lstep0 = S.lstep[0]; W.lstep1 = S.lstep[1]; W.lstep2 = S.lstep[2];
+    W.h0 = S.h[0]; W.h1 = S.h[1]; W.h2 = S.h[2];
+}
+
+// Stream a runs over plane indices kn, kn + kdir, ... up to (excluding) kstop.
+__device__ __forceinline__ void set_stream(const GridK& g, StreamInit& S, int a, int kn, int kstop, int kdir) {
+    S.ln[a] = (float)(2 * kn - g.n[a]);
+    S.lstop[a] = (float)(2 * kstop - g.n[a]);
+    S.lstep[a] = kdir > 0 ? 2.0f : -2.0f;
 }
 
 // Position all streams at their first element (walk from the ray start).
 __device__ __forceinline__ void walker_begin(const GridK& g, const Ray& R, Walker& W) {
     W.remaining = 2;
+    StreamInit S;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (R.dST[a] > 0.0f) { W.kn[a] = R.k0[a]; W.kstop[a] = R.k1[a]; W.kdir[a] = 1; }
-        else { W.kn[a] = R.k1[a] - 1; W.kstop[a] = R.k0[a] - 1; W.kdir[a] = -1; }
-        if (R.dST[a] == 0.0f) { W.kn[a] = 0; W.kstop[a] = 0; W.kdir[a] = 1; }
-        W.remaining += stream_len(W, a);
-        W.h[a] = W.kn[a] != W.kstop[a] ? plane_alpha(g, R, a, W.kn[a]) : VIR_INF;
+        int kn, ks, kd;
+        if (R.dST[a] > 0.0f) { kn = R.k0[a]; ks = R.k1[a]; kd = 1; }
+        else { kn = R.k1[a] - 1; ks = R.k0[a] - 1; kd = -1; }
+        if (R.dST[a] == 0.0f) { kn = 0; ks = 0; kd = 1; }
+        set_stream(g, S, a, kn, ks, kd);
+        W.remaining += kd > 0 ? ks - kn : kn - ks;
+        S.h[a] = kn != ks ? plane_alpha(g, R, a, kn) : VIR_INF;
     }
+    walker_store(W, S);
     W.spec = 0;
     W.sp = R.amin;
     W.consumed = 0;
@@ -342,45 +375,52 @@ __device__ __forceinline__ void walker_begin(const GridK& g, const Ray& R, Walke
 // coordinate needs no per-axis selection.
 template <bool ISO, bool FAST>
 __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker& W) {
-    float h0 = W.h[0], h1 = W.h[1], h2 = W.h[2];
+    float h0 = W.h0, h1 = W.h1, h2 = W.h2;
     bool c1 = h1 < h0;
     float hm = c1 ? h1 : h0;
     bool c2 = h2 < hm;
     hm = c2 ? h2 : hm;
-    int am = c2 ? 2 : (c1 ? 1 : 0);
     W.remaining -= 1;
     // the special value goes first on ties.  Once both are consumed sp is
     // +inf; on the FAST path every plane alpha is finite (|q| <= 2^96), so
     // while elements remain hm < sp.  The general path keeps the count test.
-    if ((FAST || W.spec < 2) && !(hm < W.sp)) {
-        float sv = W.sp;
-        W.spec += 1;
-        W.sp = W.spec == 1 ? R.amax : VIR_INF;
-        return sv;
-    }
-    bool s0 = am == 0, s1 = am == 1, s2 = am == 2;
-    int kd = s2 ? W.kdir[2] : (s1 ? W.kdir[1] : W.kdir[0]);
-    int k = (s2 ? W.kn[2] : (s1 ? W.kn[1] : W.kn[0])) + kd;
-    int ks = s2 ? W.kstop[2] : (s1 ? W.kstop[1] : W.kstop[0]);
+    // Branch-free: the plane step is computed either way and only committed
+    // when a plane is taken.
+    const bool sp_take = (FAST || W.spec < 2) && !(hm < W.sp);
+    // stream choice straight from the two comparisons: c2 -> axis 2,
+    // else c1 -> axis 1, else axis 0
+    const bool s2 = c2 && !sp_take, s1 = !c2 && c1 && !sp_take, s0 = !c2 && !c1 && !sp_take;
+    // (picks among values, not among members: a conditional over lvalues
+    // would pick an address)
+    const float ln0 = W.ln0, ln1 = W.ln1, ln2 = W.ln2;
+    const float st0 = W.lstep0, st1 = W.lstep1, st2 = W.lstep2;
+    const float ls0 = W.lstop0, ls1 = W.lstop1, ls2 = W.lstop2;
+    const float ws0 = R.s[0], ws1 = R.s[1], ws2 = R.s[2];
+    const float we0 = R.dST[0], we1 = R.dST[1], we2 = R.dST[2];
+    const float wy0 = R.y[0], wy1 = R.y[1], wy2 = R.y[2];
+    float l = (c2 ? ln2 : (c1 ? ln1 : ln0)) + (c2 ? st2 : (c1 ? st1 : st0));
+    float ls = c2 ? ls2 : (c1 ? ls1 : ls0);
     float pc;
     if (ISO) {
-        pc = plane_at(g.n[0], g.d[0], g.o[0], k);
+        pc = plane_lin(g.d[0], g.o[0], l);
     } else {
-        pc = plane_at(s2 ? g.n[2] : (s1 ? g.n[1] : g.n[0]), s2 ? g.d[2] : (s1 ? g.d[1] : g.d[0]),
-                      s2 ? g.o[2] : (s1 ? g.o[1] : g.o[0]), k);
+        pc = plane_lin(c2 ? g.d[2] : (c1 ? g.d[1] : g.d[0]), c2 ? g.o[2] : (c1 ? g.o[1] : g.o[0]), l);
     }
-    float sa = s2 ? R.s[2] : (s1 ? R.s[1] : R.s[0]);
-    float da = s2 ? R.dST[2] : (s1 ? R.dST[1] : R.dST[0]);
-    float ya = s2 ? R.y[2] : (s1 ? R.y[1] : R.y[0]);
+    float sa = c2 ? ws2 : (c1 ? ws1 : ws0);
+    float da = c2 ? we2 : (c1 ? we1 : we0);
+    float ya = c2 ? wy2 : (c1 ? wy1 : wy0);
     float hn = div_rn_t<FAST>(pc - sa, da, ya);
-    hn = k != ks ? hn : VIR_INF;
-    W.kn[0] = s0 ? k : W.kn[0];
-    W.kn[1] = s1 ? k : W.kn[1];
-    W.kn[2] = s2 ? k : W.kn[2];
-    W.h[0] = s0 ? hn : h0;
-    W.h[1] = s1 ? hn : h1;
-    W.h[2] = s2 ? hn : h2;
-    return hm;
+    hn = l != ls ? hn : VIR_INF;
+    W.ln0 = s0 ? l : ln0;
+    W.ln1 = s1 ? l : ln1;
+    W.ln2 = s2 ? l : ln2;
+    W.h0 = s0 ? hn : h0;
+    W.h1 = s1 ? hn : h1;
+    W.h2 = s2 ? hn : h2;
+    const float sv = W.sp;
+    W.spec += sp_take ? 1 : 0;
+    W.sp = sp_take ? (W.spec == 1 ? R.amax : VIR_INF) : sv;
+    return sp_take ? sv : hm;
 }
 
 // Position all streams just after every element <= T (used to enter a ray
@@ -391,11 +431,13 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
     W.remaining = 2;
     int cnt = 0;
     float pv = -VIR_INF;
+    StreamInit S;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         int lo = R.k0[a], hi = R.k1[a];
         if (R.dST[a] == 0.0f || lo >= hi) {
-            W.kn[a] = 0; W.kstop[a] = 0; W.kdir[a] = 1; W.h[a] = VIR_INF;
+            set_stream(g, S, a, 0, 0, 1);
+            S.h[a] = VIR_INF;
             continue;
         }
         float pos = R.s[a] + T * R.dST[a];
@@ -421,7 +463,7 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
             int taken = k - lo;
             if (taken > 0) pv = fmaxf(pv, am1);
             cnt += taken;
-            W.kn[a] = k; W.kstop[a] = hi; W.kdir[a] = 1;
+            set_stream(g, S, a, k, hi, 1);
             W.remaining += hi - k;
         } else {
             // stream runs hi-1 .. lo; next = largest k in [lo, hi) with alpha(k) > T
@@ -442,11 +484,12 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
             int taken = (hi - 1) - k;
             if (taken > 0) pv = fmaxf(pv, ap1);
             cnt += taken;
-            W.kn[a] = k; W.kstop[a] = lo - 1; W.kdir[a] = -1;
+            set_stream(g, S, a, k, lo - 1, -1);
             W.remaining += k - (lo - 1);
         }
-        W.h[a] = hk;
+        S.h[a] = hk;
     }
+    walker_store(W, S);
     W.spec = 0;
     if (R.amin <= T) { W.spec = 1; cnt += 1; pv = fmaxf(pv, R.amin); }
     if (W.spec == 1 && R.amax <= T) { W.spec = 2; cnt += 1; pv = fmaxf(pv, R.amax); }
@@ -505,7 +548,15 @@ __device__ __forceinline__ bool trim_close(const Ray& R, float x, float y) {
 // they are subject to the end trims.
 // FAST: every lane's ray satisfies planes_unguarded (float32 list, divisions
 // inside the guard band); see walk_dispatch.
-template <bool P2, bool ISO, bool EXACT = false, bool FAST = false, class Visit>
+// ALL: visit(ix, iy, iz, len, keep) is called for every pair, keep false for
+// the pairs the reference drops (branch-free visitors).
+template <bool ALL, class Visit>
+__device__ __forceinline__ void visit_pair(Visit& visit, const int idx[3], float len, bool keep) {
+    if constexpr (ALL) visit(idx[0], idx[1], idx[2], len, keep);
+    else if (keep) visit(idx[0], idx[1], idx[2], len);
+}
+
+template <bool P2, bool ISO, bool EXACT = false, bool FAST = false, bool ALL = false, class Visit>
 __device__ __forceinline__ void walk_entries(const GridK& g, const Ray& R, Walker& W,
                                              float a_stop, Visit&& visit) {
     int idx[3];
@@ -521,21 +572,21 @@ __device__ __forceinline__ void walk_entries(const GridK& g, const Ray& R, Walke
         bool skip = trim_close<FAST>(R, W.prev, b);                   // drop A[0]
         if (W.remaining == 0) skip |= trim_close<FAST>(R, b, W.prev);  // drop A[-1]
         bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
-        if (keep && !skip) visit(idx[0], idx[1], idx[2], len);
+        visit_pair<ALL>(visit, idx, len, keep && !skip);
         W.prev = b;
         W.consumed = 2;
     }
     while (W.remaining > 1 && !(W.prev > a_stop)) {
         float b = walker_pop<ISO, FAST>(g, R, W);
         bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
-        if (keep) visit(idx[0], idx[1], idx[2], len);
+        visit_pair<ALL>(visit, idx, len, keep);
         W.prev = b;
     }
     if (W.remaining == 1 && !(W.prev > a_stop)) {
         float b = walker_pop<ISO, FAST>(g, R, W);
         bool skip = trim_close<FAST>(R, b, W.prev);                    // drop A[-1]
         bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
-        if (keep && !skip) visit(idx[0], idx[1], idx[2], len);
+        visit_pair<ALL>(visit, idx, len, keep && !skip);
         W.prev = b;
     }
 }
