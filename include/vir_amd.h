@@ -87,7 +87,7 @@ int vir_siddon_count_guarded(const float *src, const float *trg, int64_t nrays,
  * ray's Siddon entries, with no stored matrix.  Same result as
  * siddons(flat=True) followed by the CSR gather of
  * projection/proj_numba.py:11-20 / proj_cuda.py:14-27, accumulated in float64
- * as proj_numba.py:17 does.
+ * as proj_numba.py:17 does.  The volume has fewer than 2^32 voxels.
  * err_count (may be NULL) receives the number of out-of-grid entries. */
 int vir_siddon_fp(const float *src, const float *trg, int64_t nrays,
                   const vir_grid *grid, const float *vol, float *sino,

@@ -134,7 +134,9 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
                               (unsigned)iz < (unsigned)g.n[2];
                     bad += (keep && !in) ? 1 : 0;
                     bool use = keep && in;
-                    unsigned vi = (unsigned)ix * nyz + (unsigned)iy * (unsigned)g.n[2] + (unsigned)iz;
+                    // iy, n2 <= 2^22 and iy n2 < n1 n2: a 24-bit (full-rate)
+                    // multiply is exact for in-grid entries
+                    unsigned vi = (unsigned)ix * nyz + __umul24((unsigned)iy, (unsigned)g.n[2]) + (unsigned)iz;
                     sum += (double)((pin ? pv : 0.0f) * pl);
                     pv = vol[use ? vi : 0u];
                     pin = use;
@@ -476,6 +478,8 @@ int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_g
     int rc = make_grid(grid, g);
     if (rc) return rc;
     if ((rc = check_rays(src, trg, nrays))) return rc;
+    if ((int64_t)g.n[0] * g.n[1] * g.n[2] >= ((int64_t)1 << 32))
+        return fail_arg("volume must have fewer than 2^32 voxels");
     if (nrays == 0) return VIR_OK;
     if (!vol || !sino) return fail_arg("vol/sino is NULL");
     int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;

@@ -323,6 +323,7 @@ struct Walker {
     float h0, h1, h2;           // alpha of the next plane, +inf once exhausted
     int spec;      // 0: alpha_min pending, 1: alpha_max pending, 2: none
     float sp;      // the pending special value (alpha_min, alpha_max or +inf)
+    float sp2;     // the special value after it (alpha_max or +inf; FAST pops)
     int remaining; // elements not yet consumed
     float prev;    // last consumed element
     int consumed;  // elements consumed so far, saturating at 2
@@ -365,6 +366,7 @@ __device__ __forceinline__ void walker_begin(const GridK& g, const Ray& R, Walke
     walker_store(W, S);
     W.spec = 0;
     W.sp = R.amin;
+    W.sp2 = R.amax;
     W.consumed = 0;
     W.prev = 0.0f;
 }
@@ -419,7 +421,12 @@ __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker
     W.h2 = s2 ? hn : h2;
     const float sv = W.sp;
     W.spec += sp_take ? 1 : 0;
-    W.sp = sp_take ? (W.spec == 1 ? R.amax : VIR_INF) : sv;
+    if constexpr (FAST) {
+        W.sp = sp_take ? W.sp2 : sv;
+        W.sp2 = sp_take ? VIR_INF : W.sp2;
+    } else {
+        W.sp = sp_take ? (W.spec == 1 ? R.amax : VIR_INF) : sv;
+    }
     return sp_take ? sv : hm;
 }
 
@@ -494,6 +501,7 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
     if (R.amin <= T) { W.spec = 1; cnt += 1; pv = fmaxf(pv, R.amin); }
     if (W.spec == 1 && R.amax <= T) { W.spec = 2; cnt += 1; pv = fmaxf(pv, R.amax); }
     W.sp = W.spec == 0 ? R.amin : (W.spec == 1 ? R.amax : VIR_INF);
+    W.sp2 = W.spec == 0 ? R.amax : VIR_INF;
     W.remaining -= W.spec;
     W.consumed = cnt > 2 ? 2 : cnt;
     W.prev = pv;
