@@ -94,7 +94,7 @@ int vir_siddon_fp(const float *src, const float *trg, int64_t nrays,
                   uint32_t flags, int32_t *err_count, void *stream);
 
 /* Exact adjoint of vir_siddon_fp for rays laid out as
- * (nviews, nu, nv) -- ray (a, iu, iv) at index (a*nu + iu)*nv + iv --
+ * (nviews, nu, nv), nu*nv < 2^30 -- ray (a, iu, iv) at index (a*nu + iu)*nv + iv --
  * whose view geometry is given by per-view 3x4 projection matrices
  * proj[a*12 .. a*12+11] mapping a world point X to the fractional ray
  * coordinates (iu, iv) = (P0.X~ / P2.X~, P1.X~ / P2.X~) with P2.X~ > 0 in

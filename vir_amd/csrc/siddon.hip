@@ -217,6 +217,13 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
 
     for (int va = a_begin; va < a_end; ++va) {
         const float* P = proj + 12 * va;
+        // this view's rays: ray (va, iu, iv) at in-view offset iu*nv + iv
+        // (the host checks nu nv < 2^30, so 24-bit factors suffice and
+        // 3 (iu nv + iv) fits 32 bits)
+        const int64_t vbase = (int64_t)va * nu * nv;
+        const float* __restrict__ sino_v = sino + vbase;
+        const float* __restrict__ src_v = src + 3 * vbase;
+        const float* __restrict__ trg_v = trg + 3 * vbase;
         // lanes: the 4x4x4 lattice of corner-voxel corners (planes b0, b0+1,
         // b1-1, b1 per axis); lanes 0..7 also project the guard-box corners
         int li = (lane >> 4) & 3, lj = (lane >> 2) & 3, lk = lane & 3;
@@ -298,14 +305,35 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                 int mU = us <= f.iu1 ? (f.iu1 - us) / f.su + 1 : 0;
                 int mV = vs <= f.iv1 ? (f.iv1 - vs) / f.sv + 1 : 0;
                 int M = mU * mV;
+                // q / mV as floor((q + 0.5) RN(1/mV)): the quotient's fraction
+                // lies in [0.5/mV, 1 - 0.5/mV] and the two roundings move it by
+                // less than (q + 0.5)/mV 2^-23 < 0.5/mV while q < 2^22
+                const bool fq = M < (1 << 22);
+                const float inv_mV = 1.0f / (float)mV;
                 for (int q = lane; q < M; q += 64) {
-                    int iu = us + (q / mV) * f.su;
-                    int iv = vs + (q % mV) * f.sv;
-                    int64_t ray = ((int64_t)va * nu + iu) * nv + iv;
-                    float wgt = sino[ray];
+                    int qu, qv;
+                    if (fq) {
+                        qu = (int)(((float)q + 0.5f) * inv_mV);
+                        qv = q - (int)__umul24((unsigned)qu, (unsigned)mV);
+                    } else {
+                        qu = q / mV;
+                        qv = q % mV;
+                    }
+                    int iu = us + qu * f.su;
+                    int iv = vs + qv * f.sv;
+                    const unsigned ro = __umul24((unsigned)iu, (unsigned)nv) + (unsigned)iv;
+                    // the endpoint loads go out with the weight's (the empty
+                    // asm keeps them above the test): one memory round trip
+                    // per candidate
+                    float wgt = sino_v[ro];
+                    const float* sr = src_v + 3 * ro;
+                    const float* tr = trg_v + 3 * ro;
+                    f3v sp = {sr[0], sr[1], sr[2]};
+                    f3v tp = {tr[0], tr[1], tr[2]};
+                    asm volatile("" : "+v"(sp), "+v"(tp));
                     if (wgt == 0.0f) continue;
                     Ray R;
-                    ray_setup_core<UNG>(g, src + 3 * ray, trg + 3 * ray, R);
+                    ray_setup_core_v<UNG>(g, sp, tp, R);
                     if (!R.hit) continue;
                     // alpha interval of the ray inside the guard box
                     float alo = -VIR_INF, ahi = VIR_INF;
@@ -533,6 +561,7 @@ int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu
     int rc = make_grid(grid, g);
     if (rc) return rc;
     if (nviews < 1 || nu < 1 || nv < 1) return fail_arg("nviews/nu/nv must be >= 1");
+    if ((int64_t)nu * nv >= ((int64_t)1 << 30)) return fail_arg("nu*nv must be < 2^30");
     if (!src || !trg || !proj || !sino || !vol) return fail_arg("NULL pointer argument");
     BpPlan p = plan_bp(g, nviews);
     int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];

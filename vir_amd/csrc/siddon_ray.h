@@ -142,13 +142,13 @@ struct Ray {
 // left at the full plane range [0, n+1).
 // FAST: the caller has established entry_unguarded for this ray, so the six
 // entry/exit divisions return the same bits without div_rn's range check.
+// The endpoints come as values (ray_setup_core_v) or are loaded here.
 template <bool FAST = false>
-__device__ __forceinline__ void ray_setup_core(const GridK& g, const float* __restrict__ src,
-                                               const float* __restrict__ trg, Ray& R) {
+__device__ __forceinline__ void ray_setup_core_v(const GridK& g, f3v sp, f3v tp, Ray& R) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        R.s[a] = src[a];
-        R.dST[a] = trg[a] - R.s[a];
+        R.s[a] = sp[a];
+        R.dST[a] = tp[a] - R.s[a];
         R.y[a] = 1.0f / R.dST[a];
     }
     float sq0 = R.dST[0] * R.dST[0];
@@ -182,6 +182,14 @@ __device__ __forceinline__ void ray_setup_core(const GridK& g, const float* __re
     R.amin = amin;
     R.amax = amax;
     R.hit = (amax > amin) && !isclose_f(amax, amin, g.eps_f);
+}
+
+template <bool FAST = false>
+__device__ __forceinline__ void ray_setup_core(const GridK& g, const float* __restrict__ src,
+                                               const float* __restrict__ trg, Ray& R) {
+    f3v sp = {src[0], src[1], src[2]};
+    f3v tp = {trg[0], trg[1], trg[2]};
+    ray_setup_core_v<FAST>(g, sp, tp, R);
 }
 
 // The snapped plane-index bounds i0/iN of siddon.py:740-751 as python slice
@@ -274,24 +282,22 @@ __device__ __forceinline__ bool planes_unguarded(const GridK& g, const Ray& R) {
     return ok;
 }
 
-// True when the alpha window [t0, t1] lies at least three planes inside the
-// ray's entry and exit on every axis it crosses.  The snapped slice bounds
-// differ from the geometric entry/exit planes by at most one plane, so a walk
-// confined to such a window meets the same elements with or without them.
+// True when the alpha window [t0, t1] lies at least four planes inside the
+// ray's entry and exit on every axis it crosses.  On axis a the plane
+// coordinate is affine in alpha with slope dST/d, so the condition is
+// (t0 - alpha_min) |dST| / d >= 4 and (alpha_max - t1) |dST| / d >= 4 for the
+// smallest |dST| / d; it is tested against 4.5, which covers the rounding of
+// these few float operations.  The snapped slice bounds differ from the
+// geometric entry/exit planes by at most one plane, so a walk confined to
+// such a window meets the same elements with or without them.  NaN fails.
 __device__ __forceinline__ bool window_interior(const GridK& g, const Ray& R, float t0, float t1) {
-    bool ok = true;
+    float gmin = VIR_INF;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (R.dST[a] == 0.0f) continue;
-        float ui = (R.s[a] + R.amin * R.dST[a] - g.p0[a]) * g.inv_d[a];
-        float uo = (R.s[a] + R.amax * R.dST[a] - g.p0[a]) * g.inv_d[a];
-        float w0 = (R.s[a] + t0 * R.dST[a] - g.p0[a]) * g.inv_d[a];
-        float w1 = (R.s[a] + t1 * R.dST[a] - g.p0[a]) * g.inv_d[a];
-        float rlo = fminf(ui, uo) + 3.0f, rhi = fmaxf(ui, uo) - 3.0f;
-        float wlo = fminf(w0, w1) - 1.0f, whi = fmaxf(w0, w1) + 1.0f;
-        ok = ok && (wlo >= rlo) && (whi <= rhi);
+        float ga = fabsf(R.dST[a]) * g.inv_d[a];
+        gmin = R.dST[a] != 0.0f ? fminf(gmin, ga) : gmin;
     }
-    return ok;
+    return (t0 - R.amin) * gmin >= 4.5f && (R.amax - t1) * gmin >= 4.5f;
 }
 
 template <bool FAST = false>
