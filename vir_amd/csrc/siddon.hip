@@ -322,16 +322,14 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     int iu = us + qu * f.su;
                     int iv = vs + qv * f.sv;
                     const unsigned ro = __umul24((unsigned)iu, (unsigned)nv) + (unsigned)iv;
-                    // the endpoint loads go out with the weight's (the empty
-                    // asm keeps them above the test): one memory round trip
-                    // per candidate
+                    // zero-weight rays (they miss the object) skip the
+                    // endpoint loads altogether
                     float wgt = sino_v[ro];
+                    if (wgt == 0.0f) continue;
                     const float* sr = src_v + 3 * ro;
                     const float* tr = trg_v + 3 * ro;
                     f3v sp = {sr[0], sr[1], sr[2]};
                     f3v tp = {tr[0], tr[1], tr[2]};
-                    asm volatile("" : "+v"(sp), "+v"(tp));
-                    if (wgt == 0.0f) continue;
                     Ray R;
                     ray_setup_core_v<UNG>(g, sp, tp, R);
                     if (!R.hit) continue;
