@@ -98,6 +98,20 @@ __device__ __forceinline__ float dd_invert(const DDGeo& G, const DDView& V, floa
     return (t * (G.DSO - oo) - po) / (V.pd + t * V.od);
 }
 
+// dd_invert as x = A - oc B for a fixed slope t (the tiled forward's range).
+template <int MODE>
+__device__ __forceinline__ void dd_invert_lin(const DDGeo& G, const DDView& V, float t, float& A, float& B) {
+    if (MODE == DD_PAR) {
+        const float r = 1.0f / V.pd;
+        A = t * r;
+        B = V.po * r;
+    } else {
+        const float r = 1.0f / (V.pd + t * V.od);
+        A = t * G.DSO * r;
+        B = (t * V.oo + V.po) * r;
+    }
+}
+
 __device__ __forceinline__ float overlap(float a0, float a1, float b0, float b1) {
     float lo = a0 < a1 ? a0 : a1, hi = a0 < a1 ? a1 : a0;
     return fminf(hi, b1) - fmaxf(lo, b0);
@@ -172,6 +186,13 @@ __global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __re
     const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
     const float ul = det_bnd(G.nu, G.du, G.su, iu), ur = det_bnd(G.nu, G.du, G.su, iu + 1);
     const float tl = dd_slope<MODE>(G, ul), tr = dd_slope<MODE>(G, ur);
+    // dd_invert(t, oc) = A(t) - oc B(t): its denominator depends on the bin
+    // only.  The inverse only bounds the pixel range, which is widened by a
+    // pixel, so this rounding (~1e-5 pixel) leaves the range covering every
+    // pixel of positive overlap and the sum unchanged.
+    float Al, Bl, Ar, Br;
+    dd_invert_lin<MODE>(G, V, tl, Al, Bl);
+    dd_invert_lin<MODE>(G, V, tr, Ar, Br);
     const float tt0 = dd_slope<MODE>(G, det_bnd(G.nu, G.du, G.su, u_lo));
     const float tt1 = dd_slope<MODE>(G, det_bnd(G.nu, G.du, G.su, u_lo + u_n));
     const float inv_d = 1.0f / G.d;
@@ -206,7 +227,7 @@ __global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __re
     // this bin's pixels of line io within the staged chunk, as k_dd2d_fp
     auto merge = [&](int h, int io, int kc, int nk) {
         const float oc = cell_cen(no, G.d, io);
-        const float x0 = dd_invert<MODE>(G, V, tl, oc), x1 = dd_invert<MODE>(G, V, tr, oc);
+        const float x0 = __builtin_fmaf(-oc, Bl, Al), x1 = __builtin_fmaf(-oc, Br, Ar);
         const float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd;
         const float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd;
         if (!(k0f < (float)nd + 2.0f) || !(k1f > -2.0f)) return;
