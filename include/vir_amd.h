@@ -111,6 +111,26 @@ int vir_siddon_bp(const float *src, const float *trg, int32_t nviews,
                   const vir_grid *grid, const float *sino, float *vol,
                   uint32_t flags, void *workspace, void *stream);
 
+/* Per-ray records for vir_siddon_bp_rec: VIR_SIDDON_RECORD_BYTES per ray
+ * (rec 16-byte aligned), the ray's setup as the forward projector computes
+ * it -- direction, reciprocals, length, entry/exit alphas, gate and the exact
+ * plane slices of siddon.py:705-760.  Built once per ray set and grid.  No
+ * reference counterpart: the per-geometry part of the trace, precomputed as
+ * the reference precomputes its whole system matrix (siddons(), ~1 KB per
+ * ray on the 256^3 cone scan) but at 80 bytes per ray. */
+#define VIR_SIDDON_RECORD_BYTES 80
+int vir_siddon_ray_records(const float *src, const float *trg, int64_t nrays,
+                           const vir_grid *grid, float *rec, void *stream);
+
+/* vir_siddon_bp for rays given by their records (same layout, arguments and
+ * result, bit for bit): the gather reads each candidate ray's record instead
+ * of recomputing its setup.  Replaces the same reference interfaces as
+ * vir_siddon_bp. */
+int vir_siddon_bp_rec(const float *rec, int32_t nviews, int32_t nu, int32_t nv,
+                      const float *proj, const vir_grid *grid,
+                      const float *sino, float *vol, uint32_t flags,
+                      void *workspace, void *stream);
+
 /* ------------------------------------------------- stored matrix (CSR) */
 
 /* sino[r] = sum_{e in [off[r], off[r+1])} vol[ind[e]] * len[e]

@@ -53,6 +53,19 @@ def test_invalid_arguments_rejected():
     assert lib.vir_siddon_count_guarded(None, None, 0, g, None, None) == -1  # count is NULL
     assert lib.vir_csr_transpose(None, None, None, 1, 1 << 31, 8, None, None, None, None) == -1
     assert lib.vir_siddon_bp_workspace(g, 0) == -1
+    # per-ray records: NULL endpoints / records, misaligned records, big detector
+    assert lib.vir_siddon_ray_records(None, None, 1, g, None, None) == -1
+    assert lib.vir_siddon_ray_records(None, None, 0, g, None, None) == 0  # nothing to do
+    assert lib.vir_siddon_ray_records(ctypes.c_void_p(64), ctypes.c_void_p(64), 1, g, None, None) == -1
+    assert lib.vir_siddon_ray_records(ctypes.c_void_p(64), ctypes.c_void_p(64), 1, g, ctypes.c_void_p(72),
+                                      None) == -1
+    assert b"aligned" in lib.vir_last_error()
+    assert lib.vir_siddon_bp_rec(None, 1, 1, 1, None, g, None, None, 0, None, None) == -1
+    assert lib.vir_siddon_bp_rec(ctypes.c_void_p(72), 1, 1, 1, ctypes.c_void_p(64), g, ctypes.c_void_p(64),
+                                 ctypes.c_void_p(64), 0, None, None) == -1
+    assert lib.vir_siddon_bp_rec(ctypes.c_void_p(64), 1, 1 << 15, 1 << 15, ctypes.c_void_p(64), g,
+                                 ctypes.c_void_p(64), ctypes.c_void_p(64), 0, None, None) == -1
+    assert b"nu*nv" in lib.vir_last_error()
 
 
 @pytest.mark.parametrize("name,kw", [

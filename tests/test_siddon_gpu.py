@@ -197,6 +197,53 @@ def test_unguarded_kernels_same_bits(kind):
     assert np.array_equal(outs[0][1].view(np.int32), outs[1][1].view(np.int32))
 
 
+@pytest.mark.parametrize("kind", ["par2d", "fan2d", "cone3d", "par3d", "cone256"])
+def test_bp_from_records_same_bits(kind):
+    """vir_siddon_bp_rec, reading the per-ray records that
+    vir_siddon_ray_records builds once, equals vir_siddon_bp, which recomputes
+    each candidate ray's setup (and its exact plane slices only near the
+    ray's ends), bit for bit: overwrite and VIR_ACCUMULATE, with and without
+    VIR_UNGUARDED where its precondition holds.  cone256: 8 views of the
+    256^3 cone-beam workload."""
+    import torch
+    from vir_amd import _lib
+    from vir_amd.device import stream
+    sd = _sd()
+    rng = np.random.default_rng(12)
+    if kind == "cone256":
+        from vir_amd.workloads import cone256_geometry
+        s, t, grid = cone256_geometry(np.arange(0, 360, 45))
+        nP, dP = grid["nPixels"], grid["dPixels"]
+    else:
+        s, t, nP, dP = _geometry(kind, rng)
+    print("case", kind, flush=True)
+    P = sd.SiddonProjector(s, t, nP, dP)
+    assert P.rec is not None and P.rec.numel() == P.nrays * 20
+    assert sd.SiddonProjector(s, t, nP, dP, records=False).rec is None
+    L = _lib.lib()
+    y = torch.from_numpy(rng.uniform(-1, 1, P.nrays).astype(np.float32)).cuda()
+    z = torch.from_numpy(rng.uniform(0, 1, P.nvox).astype(np.float32)).cuda()
+    nviews, nu, nv = P.layout
+    for fl in sorted({0, P._ung}):
+        for acc in (0, _lib.VIR_ACCUMULATE):
+            a, b = z.clone(), z.clone()
+            _lib.check(L.vir_siddon_bp(_lib.ptr(P.src), _lib.ptr(P.trg), nviews, nu, nv, _lib.ptr(P.proj),
+                                       P.grid, _lib.ptr(y), _lib.ptr(a), fl | acc, _lib.ptr(P._ws), stream()),
+                       "bp")
+            _lib.check(L.vir_siddon_bp_rec(_lib.ptr(P.rec), nviews, nu, nv, _lib.ptr(P.proj), P.grid,
+                                           _lib.ptr(y), _lib.ptr(b), fl | acc, _lib.ptr(P._ws), stream()),
+                       "bp_rec")
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            assert np.abs(a).max() > 0
+            assert np.array_equal(a.view(np.int32), b.view(np.int32)), (kind, fl, acc)
+    # the projector's own bp() takes the record path
+    c = P.bp(y).cpu().numpy().reshape(-1)
+    a = torch.empty(P.nvox, dtype=torch.float32, device="cuda")
+    _lib.check(L.vir_siddon_bp(_lib.ptr(P.src), _lib.ptr(P.trg), nviews, nu, nv, _lib.ptr(P.proj), P.grid,
+                               _lib.ptr(y), _lib.ptr(a), P._ung, _lib.ptr(P._ws), stream()), "bp")
+    assert np.array_equal(c.view(np.int32), a.cpu().numpy().view(np.int32))
+
+
 @pytest.mark.parametrize("kind", ["par2d", "cone3d"])
 def test_bp_single_rays_exact(kind):
     """Backprojecting a one-hot sinogram reproduces that ray's own entries

@@ -16,6 +16,7 @@ VIR_ACCUMULATE = 1
 VIR_SEQUENTIAL = 2
 VIR_UNGUARDED = 4
 VIR_PER_OUTPUT = 8
+SIDDON_RECORD_BYTES = 80  # include/vir_amd.h: VIR_SIDDON_RECORD_BYTES
 
 P = ctypes.c_void_p
 I64 = ctypes.c_int64
@@ -89,6 +90,8 @@ SIGNATURES = {
     "vir_siddon_count_guarded": (ctypes.c_int, [P, P, I64, G, P, P]),
     "vir_siddon_bp_workspace": (I64, [G, I32]),
     "vir_siddon_bp": (ctypes.c_int, [P, P, I32, I32, I32, P, G, P, P, U32, P, P]),
+    "vir_siddon_ray_records": (ctypes.c_int, [P, P, I64, G, P, P]),
+    "vir_siddon_bp_rec": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P, P]),
     "vir_csr_fp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
     "vir_csr_transpose": (ctypes.c_int, [P, P, P, I64, I64, I64, P, P, P, P]),
     "vir_csc_bp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <vector>
 
 #include "vir_common.h"
@@ -100,6 +101,20 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_count_guarded(GridK g, c
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int)__popcll(m));
 }
 
+// Per-ray records for the backprojector: ray_setup (the forward projector's
+// per-ray setup and exact slices) stored as ray_store_record lays it out.
+__global__ __launch_bounds__(kRayThreads) void k_siddon_ray_records(GridK g, const float* __restrict__ src,
+                                                                   const float* __restrict__ trg,
+                                                                   int64_t nrays, float* __restrict__ rec) {
+    int64_t r = (int64_t)blockIdx.x * kRayThreads + threadIdx.x;
+    if (r >= nrays) return;
+    Ray R;
+    ray_setup_core(g, src + 3 * r, trg + 3 * r, R);
+    if (R.hit) ray_setup_slices(g, R);
+    else { R.k0 = i3v{0, 0, 0}; R.k1 = i3v{0, 0, 0}; }
+    ray_store_record(R, rec + kRayRec * r);
+}
+
 // ------------------------------------------------------------ forward
 
 template <bool P2, bool ISO, bool UNG>
@@ -179,7 +194,8 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
 // in-order LDS pipeline -- no atomics and no workgroup barriers.
 template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, bool P2, bool ISO, bool UNG>
 __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WPE))) void k_siddon_bp_wave(GridK g, const float* __restrict__ src,
-                                                      const float* __restrict__ trg, int nviews, int nu,
+                                                      const float* __restrict__ trg,
+                                                      const float* __restrict__ rec, int nviews, int nu,
                                                       int nv, const float* __restrict__ proj,
                                                       const float* __restrict__ sino,
                                                       float* __restrict__ out, int views_per_chunk,
@@ -222,8 +238,9 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
         // 3 (iu nv + iv) fits 32 bits)
         const int64_t vbase = (int64_t)va * nu * nv;
         const float* __restrict__ sino_v = sino + vbase;
-        const float* __restrict__ src_v = src + 3 * vbase;
-        const float* __restrict__ trg_v = trg + 3 * vbase;
+        const float* __restrict__ src_v = rec ? nullptr : src + 3 * vbase;
+        const float* __restrict__ trg_v = rec ? nullptr : trg + 3 * vbase;
+        const float* __restrict__ rec_v = rec ? rec + kRayRec * vbase : nullptr;
         // lanes: the 4x4x4 lattice of corner-voxel corners (planes b0, b0+1,
         // b1-1, b1 per axis); lanes 0..7 also project the guard-box corners
         int li = (lane >> 4) & 3, lj = (lane >> 2) & 3, lk = lane & 3;
@@ -326,12 +343,17 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     // endpoint loads altogether
                     float wgt = sino_v[ro];
                     if (wgt == 0.0f) continue;
-                    const float* sr = src_v + 3 * ro;
-                    const float* tr = trg_v + 3 * ro;
-                    f3v sp = {sr[0], sr[1], sr[2]};
-                    f3v tp = {tr[0], tr[1], tr[2]};
                     Ray R;
-                    ray_setup_core_v<UNG>(g, sp, tp, R);
+                    if (rec) {
+                        // precomputed setup and exact slices (uniform branch)
+                        ray_load_record(rec_v + (size_t)kRayRec * ro, R);
+                    } else {
+                        const float* sr = src_v + 3 * ro;
+                        const float* tr = trg_v + 3 * ro;
+                        f3v sp = {sr[0], sr[1], sr[2]};
+                        f3v tp = {tr[0], tr[1], tr[2]};
+                        ray_setup_core_v<UNG>(g, sp, tp, R);
+                    }
                     if (!R.hit) continue;
                     // alpha interval of the ray inside the guard box
                     float alo = -VIR_INF, ahi = VIR_INF;
@@ -349,7 +371,7 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     }
                     if (miss || alo > ahi) continue;
                     // exact plane slices only matter near the ray's ends
-                    if (!window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
+                    if (!rec && !window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
                     walk_dispatch<UNG>(g, R, [&](auto fast) {
                         constexpr bool F = decltype(fast)::value;
                         Walker W;
@@ -552,15 +574,17 @@ int64_t vir_siddon_bp_workspace(const vir_grid* grid, int32_t nviews) {
     return (int64_t)p.nchunks * nvox * (int64_t)sizeof(float);
 }
 
-int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu, int32_t nv,
-                  const float* proj, const vir_grid* grid, const float* sino, float* vol, uint32_t flags,
-                  void* workspace, void* stream) {
+// The brick-gather backprojector from endpoints (rec == NULL) or from
+// per-ray records (src == trg == NULL).
+static int siddon_bp(const float* src, const float* trg, const float* rec, int32_t nviews, int32_t nu,
+                     int32_t nv, const float* proj, const vir_grid* grid, const float* sino, float* vol,
+                     uint32_t flags, void* workspace, void* stream) {
     GridK g;
     int rc = make_grid(grid, g);
     if (rc) return rc;
     if (nviews < 1 || nu < 1 || nv < 1) return fail_arg("nviews/nu/nv must be >= 1");
     if ((int64_t)nu * nv >= ((int64_t)1 << 30)) return fail_arg("nu*nv must be < 2^30");
-    if (!src || !trg || !proj || !sino || !vol) return fail_arg("NULL pointer argument");
+    if (!(rec || (src && trg)) || !proj || !sino || !vol) return fail_arg("NULL pointer argument");
     BpPlan p = plan_bp(g, nviews);
     int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
     hipStream_t st = as_stream(stream);
@@ -593,7 +617,7 @@ int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu
         if (ung) VIR_BP_LAUNCH1(SHAPE, true);                                                                    \
         else VIR_BP_LAUNCH1(SHAPE, false);                                                                       \
     } while (0)
-#define VIR_BP_ARGS g, src, trg, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
+#define VIR_BP_ARGS g, src, trg, rec, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
 #define BP3_ARGS 16, 16, 8, 2, 2, 4
 #define BP3_WX BpShape3::wx
 #define BP3_WY BpShape3::wy
@@ -620,6 +644,36 @@ int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu
         rc = check_hip(hipGetLastError(), "vir_siddon_bp reduce launch");
     }
     return rc;
+}
+
+int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu, int32_t nv,
+                  const float* proj, const vir_grid* grid, const float* sino, float* vol, uint32_t flags,
+                  void* workspace, void* stream) {
+    if (!src || !trg) return fail_arg("NULL pointer argument");
+    return siddon_bp(src, trg, nullptr, nviews, nu, nv, proj, grid, sino, vol, flags, workspace, stream);
+}
+
+int vir_siddon_bp_rec(const float* rec, int32_t nviews, int32_t nu, int32_t nv, const float* proj,
+                      const vir_grid* grid, const float* sino, float* vol, uint32_t flags, void* workspace,
+                      void* stream) {
+    if (!rec) return fail_arg("NULL pointer argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    return siddon_bp(nullptr, nullptr, rec, nviews, nu, nv, proj, grid, sino, vol, flags, workspace, stream);
+}
+
+int vir_siddon_ray_records(const float* src, const float* trg, int64_t nrays, const vir_grid* grid, float* rec,
+                           void* stream) {
+    GridK g;
+    int rc = make_grid(grid, g);
+    if (rc) return rc;
+    if ((rc = check_rays(src, trg, nrays))) return rc;
+    if (nrays == 0) return VIR_OK;
+    if (!rec) return fail_arg("rec is NULL");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
+    hipLaunchKernelGGL(k_siddon_ray_records, dim3((unsigned)nb), dim3(kRayThreads), 0, as_stream(stream), g, src,
+                       trg, nrays, rec);
+    return check_hip(hipGetLastError(), "vir_siddon_ray_records launch");
 }
 
 }  // extern "C"

@@ -224,6 +224,40 @@ __device__ __forceinline__ void ray_setup(const GridK& g, const float* __restric
     ray_setup_slices(g, R);
 }
 
+// Per-ray record (kRayRec floats = 80 bytes, 16-byte aligned): the outputs
+// of ray_setup -- the per-ray setup and the exact plane slices -- so a
+// backprojector that reads it instead of the endpoints traces the same
+// elements.
+//   [0..2] s  [3..5] dST  [6..8] y = RN(1/dST)  [9] dist  [10] alpha_min
+//   [11] alpha_max  [12..14] k0  [15..17] k1 (int bits)  [18] hit (0/1 int bits)
+//   [19] 0
+constexpr int kRayRec = 20;
+
+__device__ __forceinline__ void ray_store_record(const Ray& R, float* __restrict__ rec) {
+    float4* r4 = reinterpret_cast<float4*>(rec);
+    r4[0] = make_float4(R.s[0], R.s[1], R.s[2], R.dST[0]);
+    r4[1] = make_float4(R.dST[1], R.dST[2], R.y[0], R.y[1]);
+    r4[2] = make_float4(R.y[2], R.dist, R.amin, R.amax);
+    r4[3] = make_float4(__int_as_float(R.k0[0]), __int_as_float(R.k0[1]), __int_as_float(R.k0[2]),
+                        __int_as_float(R.k1[0]));
+    r4[4] = make_float4(__int_as_float(R.k1[1]), __int_as_float(R.k1[2]), __int_as_float(R.hit ? 1 : 0), 0.0f);
+}
+
+__device__ __forceinline__ void ray_load_record(const float* __restrict__ rec, Ray& R) {
+    const float4* r4 = reinterpret_cast<const float4*>(rec);
+    const float4 a = r4[0], b = r4[1], c = r4[2], d = r4[3], e = r4[4];
+    R.s = f3v{a.x, a.y, a.z};
+    R.dST = f3v{a.w, b.x, b.y};
+    R.y = f3v{b.z, b.w, c.x};
+    R.dist = c.y;
+    R.amin = c.z;
+    R.amax = c.w;
+    R.k0 = i3v{__float_as_int(d.x), __float_as_int(d.y), __float_as_int(d.z)};
+    R.k1 = i3v{__float_as_int(d.w), __float_as_int(e.x), __float_as_int(e.y)};
+    R.hit = __float_as_int(e.z) != 0;
+    R.f64 = a.w == 0.0f || b.x == 0.0f || b.y == 0.0f;
+}
+
 // True when the six entry/exit divisions of ray_setup_core, (p0 - s) / den
 // and (pN - s) / den per axis, pass div_rn's own range check, so
 // ray_setup_core<true> computes the same bits (hit or not).

@@ -298,6 +298,7 @@ class SirtBackend:
             grid = _lib.make_grid((req.n,) * 3, (1.0,) * 3, (0.0,) * 3, 1e-5)
             ws = int(_lib.lib().vir_siddon_bp_workspace(grid, req.views))
             b = 24 * nrays + 4 * 4 * nrays + 4 * 4 * nvox + max(ws, 4 * nvox) + 4 * nvox
+            b += _lib.SIDDON_RECORD_BYTES * nrays  # the backprojector's per-ray records
             b += 48 * req.views + 64
         # allocator granularity: every tensor rounded up to 2 MiB at most
         return int(b * 1.02) + 24 * (2 << 20)

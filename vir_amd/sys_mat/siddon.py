@@ -129,8 +129,11 @@ class SiddonProjector:
     """
 
     def __init__(self, src, trg, nPixels=128, dPixels=1.0, origin=0.0, epsilon=1e-5,
-                 structured=None, proj=None):
-        """proj: optional per-view projection matrices (nviews, 12) for rays
+                 structured=None, proj=None, records=True):
+        """records: build the gather backprojector's per-ray records once
+        (80 bytes per ray; vir_siddon_ray_records) so bp() reads them instead
+        of recomputing each ray's setup per call -- same bits.
+        proj: optional per-view projection matrices (nviews, 12) for rays
         already on the device (float32 CUDA tensors of shape (nviews, nu[, nv],
         3)), e.g. from vir_amd.geometry.circular_rays; the rays then stay
         on the device."""
@@ -187,8 +190,17 @@ class SiddonProjector:
         self.guarded_rays = int(guarded.item())
         self._ung = _lib.VIR_UNGUARDED if self.guarded_rays == 0 else 0
         ws = 0
+        self.rec = None
         if self.proj is not None:
             ws = int(_lib.lib().vir_siddon_bp_workspace(self.grid, self.layout[0]))
+            if records and self.nrays > 0:
+                # the per-ray setup of the gather backprojector, built once
+                # (80 bytes per ray; records=False recomputes it per call)
+                self.rec = torch.empty(self.nrays * (_lib.SIDDON_RECORD_BYTES // 4), dtype=torch.float32,
+                                       device=self.src.device)
+                _lib.check(_lib.lib().vir_siddon_ray_records(_lib.ptr(self.src), _lib.ptr(self.trg),
+                                                             self.nrays, self.grid, _lib.ptr(self.rec),
+                                                             stream()), "vir_siddon_ray_records")
         # one extra volume lets VIR_ACCUMULATE stage through the workspace
         self._ws = torch.empty(max(ws, self.nvox * 4 if self.proj is not None else 0) // 4 + 1,
                                dtype=torch.float32, device=self.src.device)
@@ -196,6 +208,7 @@ class SiddonProjector:
     def device_bytes(self):
         """Device memory held by this projector (rays, matrices, workspace)."""
         b = (self.src.numel() + self.trg.numel() + self._ws.numel()) * 4
+        b += self.rec.numel() * 4 if self.rec is not None else 0
         return b + (self.proj.numel() * 4 if self.proj is not None else 0)
 
     # -- forward ---------------------------------------------------------
@@ -230,6 +243,11 @@ class SiddonProjector:
         if self.proj is not None:
             fl |= self._ung
             nviews, nu, nv = self.layout
+            if self.rec is not None:
+                _lib.check(_lib.lib().vir_siddon_bp_rec(_lib.ptr(self.rec), nviews, nu, nv, _lib.ptr(self.proj),
+                                                        self.grid, _lib.ptr(sino), _lib.ptr(out), fl,
+                                                        _lib.ptr(self._ws), stream()), "vir_siddon_bp_rec")
+                return out
             _lib.check(_lib.lib().vir_siddon_bp(_lib.ptr(self.src), _lib.ptr(self.trg), nviews, nu,
                                                 nv, _lib.ptr(self.proj), self.grid, _lib.ptr(sino),
                                                 _lib.ptr(out), fl, _lib.ptr(self._ws), stream()),
