@@ -111,7 +111,7 @@ int vir_siddon_bp(const float *src, const float *trg, int32_t nviews,
                   const vir_grid *grid, const float *sino, float *vol,
                   uint32_t flags, void *workspace, void *stream);
 
-/* Per-ray records for vir_siddon_bp_rec: VIR_SIDDON_RECORD_BYTES per ray
+/* Per-ray records for vir_siddon_fp_rec / vir_siddon_bp_rec: VIR_SIDDON_RECORD_BYTES per ray
  * (rec 16-byte aligned), the ray's setup as the forward projector computes
  * it -- direction, reciprocals, length, entry/exit alphas, gate and the exact
  * plane slices of siddon.py:705-760.  Built once per ray set and grid.  No
@@ -121,6 +121,14 @@ int vir_siddon_bp(const float *src, const float *trg, int32_t nviews,
 #define VIR_SIDDON_RECORD_BYTES 80
 int vir_siddon_ray_records(const float *src, const float *trg, int64_t nrays,
                            const vir_grid *grid, float *rec, void *stream);
+
+/* vir_siddon_fp for rays given by their records (vir_siddon_ray_records of
+ * the same rays and grid): same result, bit for bit, without recomputing
+ * each ray's setup.  Replaces the same reference interfaces as
+ * vir_siddon_fp. */
+int vir_siddon_fp_rec(const float *rec, int64_t nrays, const vir_grid *grid,
+                      const float *vol, float *sino, uint32_t flags,
+                      int32_t *err_count, void *stream);
 
 /* vir_siddon_bp for rays given by their records (same layout, arguments and
  * result, bit for bit): the gather reads each candidate ray's record instead

@@ -61,6 +61,10 @@ def test_invalid_arguments_rejected():
                                       None) == -1
     assert b"aligned" in lib.vir_last_error()
     assert lib.vir_siddon_bp_rec(None, 1, 1, 1, None, g, None, None, 0, None, None) == -1
+    assert lib.vir_siddon_fp_rec(None, 1, g, None, None, 0, None, None) == -1
+    assert lib.vir_siddon_fp_rec(ctypes.c_void_p(72), 1, g, ctypes.c_void_p(64), ctypes.c_void_p(64), 0, None,
+                                 None) == -1
+    assert lib.vir_siddon_fp_rec(None, 0, g, None, None, 0, None, None) == 0
     assert lib.vir_siddon_bp_rec(ctypes.c_void_p(72), 1, 1, 1, ctypes.c_void_p(64), g, ctypes.c_void_p(64),
                                  ctypes.c_void_p(64), 0, None, None) == -1
     assert lib.vir_siddon_bp_rec(ctypes.c_void_p(64), 1, 1 << 15, 1 << 15, ctypes.c_void_p(64), g,

@@ -199,10 +199,10 @@ def test_unguarded_kernels_same_bits(kind):
 
 @pytest.mark.parametrize("kind", ["par2d", "fan2d", "cone3d", "par3d", "cone256"])
 def test_bp_from_records_same_bits(kind):
-    """vir_siddon_bp_rec, reading the per-ray records that
-    vir_siddon_ray_records builds once, equals vir_siddon_bp, which recomputes
-    each candidate ray's setup (and its exact plane slices only near the
-    ray's ends), bit for bit: overwrite and VIR_ACCUMULATE, with and without
+    """vir_siddon_bp_rec / vir_siddon_fp_rec, reading the per-ray records that
+    vir_siddon_ray_records builds once, equal vir_siddon_bp / vir_siddon_fp,
+    which recompute each (candidate) ray's setup, bit for bit: overwrite and
+    VIR_ACCUMULATE, with and without
     VIR_UNGUARDED where its precondition holds.  cone256: 8 views of the
     256^3 cone-beam workload."""
     import torch
@@ -236,6 +236,19 @@ def test_bp_from_records_same_bits(kind):
             a, b = a.cpu().numpy(), b.cpu().numpy()
             assert np.abs(a).max() > 0
             assert np.array_equal(a.view(np.int32), b.view(np.int32)), (kind, fl, acc)
+    # the forward projector from records: same bits and out-of-grid count
+    x = torch.from_numpy(rng.uniform(-1, 1, P.nvox).astype(np.float32)).cuda()
+    for fl in sorted({0, P._ung}):
+        for acc in (0, _lib.VIR_ACCUMULATE):
+            a, b = y.clone(), y.clone()
+            ea, eb = (torch.zeros(1, dtype=torch.int32, device="cuda") for _ in range(2))
+            _lib.check(L.vir_siddon_fp(_lib.ptr(P.src), _lib.ptr(P.trg), P.nrays, P.grid, _lib.ptr(x), _lib.ptr(a),
+                                       fl | acc, _lib.ptr(ea), stream()), "fp")
+            _lib.check(L.vir_siddon_fp_rec(_lib.ptr(P.rec), P.nrays, P.grid, _lib.ptr(x), _lib.ptr(b), fl | acc,
+                                           _lib.ptr(eb), stream()), "fp_rec")
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            assert np.array_equal(a.view(np.int32), b.view(np.int32)), (kind, fl, acc)
+            assert int(ea.item()) == int(eb.item()) == 0
     # the projector's own bp() takes the record path
     c = P.bp(y).cpu().numpy().reshape(-1)
     a = torch.empty(P.nvox, dtype=torch.float32, device="cuda")

@@ -119,7 +119,8 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_ray_records(GridK g, con
 
 template <bool P2, bool ISO, bool UNG>
 __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_siddon_fp(GridK g, const float* __restrict__ src,
-                                                          const float* __restrict__ trg, int64_t nrays,
+                                                          const float* __restrict__ trg,
+                                                          const float* __restrict__ rec, int64_t nrays,
                                                           const float* __restrict__ vol,
                                                           float* __restrict__ sino, int accumulate,
                                                           int32_t* __restrict__ err) {
@@ -127,7 +128,8 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
     int64_t r = blk * kRayThreads + threadIdx.x;
     if (r >= nrays) return;
     Ray R;
-    ray_setup<UNG>(g, src + 3 * r, trg + 3 * r, R);
+    if (rec) ray_load_record(rec + (size_t)kRayRec * r, R);  // the same setup, precomputed
+    else ray_setup<UNG>(g, src + 3 * r, trg + 3 * r, R);
     // float64 accumulation of the float32 products (the accumulator of
     // projection/proj_numba.py:17), one-deep software pipeline: the gather
     // issued for entry j lands in pv while entry j+1 is being traced, and is
@@ -520,12 +522,15 @@ int vir_siddon_fill(const float* src, const float* trg, int64_t nrays, const vir
     return check_hip(hipGetLastError(), "vir_siddon_fill launch");
 }
 
-int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_grid* grid, const float* vol,
-                  float* sino, uint32_t flags, int32_t* err_count, void* stream) {
+// The forward projector from endpoints (rec == NULL) or from per-ray
+// records (src == trg == NULL).
+static int siddon_fp(const float* src, const float* trg, const float* rec, int64_t nrays, const vir_grid* grid,
+                     const float* vol, float* sino, uint32_t flags, int32_t* err_count, void* stream) {
     GridK g;
     int rc = make_grid(grid, g);
     if (rc) return rc;
-    if ((rc = check_rays(src, trg, nrays))) return rc;
+    if (nrays < 0) return fail_arg("nrays < 0");
+    if (nrays > 0 && !(rec || (src && trg))) return fail_arg("src/trg is NULL");
     if ((int64_t)g.n[0] * g.n[1] * g.n[2] >= ((int64_t)1 << 32))
         return fail_arg("volume must have fewer than 2^32 voxels");
     if (nrays == 0) return VIR_OK;
@@ -535,7 +540,7 @@ int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_g
     const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
 #define VIR_FP_LAUNCH(P2_, ISO_, UNG_)                                                                       \
     hipLaunchKernelGGL((k_siddon_fp<P2_, ISO_, UNG_>), dim3((unsigned)nb), dim3(kRayThreads), 0,             \
-                       as_stream(stream), g, src, trg, nrays, vol, sino, acc, err_count)
+                       as_stream(stream), g, src, trg, rec, nrays, vol, sino, acc, err_count)
     if (ung) {
         if (p2 && iso) VIR_FP_LAUNCH(true, true, true);
         else if (p2) VIR_FP_LAUNCH(true, false, true);
@@ -549,6 +554,19 @@ int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_g
     }
 #undef VIR_FP_LAUNCH
     return check_hip(hipGetLastError(), "vir_siddon_fp launch");
+}
+
+int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_grid* grid, const float* vol,
+                  float* sino, uint32_t flags, int32_t* err_count, void* stream) {
+    if (nrays > 0 && (!src || !trg)) return fail_arg("src/trg is NULL");
+    return siddon_fp(src, trg, nullptr, nrays, grid, vol, sino, flags, err_count, stream);
+}
+
+int vir_siddon_fp_rec(const float* rec, int64_t nrays, const vir_grid* grid, const float* vol, float* sino,
+                      uint32_t flags, int32_t* err_count, void* stream) {
+    if (nrays > 0 && !rec) return fail_arg("rec is NULL");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    return siddon_fp(nullptr, nullptr, rec, nrays, grid, vol, sino, flags, err_count, stream);
 }
 
 int vir_siddon_count_guarded(const float* src, const float* trg, int64_t nrays, const vir_grid* grid,

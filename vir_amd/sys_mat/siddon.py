@@ -130,8 +130,8 @@ class SiddonProjector:
 
     def __init__(self, src, trg, nPixels=128, dPixels=1.0, origin=0.0, epsilon=1e-5,
                  structured=None, proj=None, records=True):
-        """records: build the gather backprojector's per-ray records once
-        (80 bytes per ray; vir_siddon_ray_records) so bp() reads them instead
+        """records: build per-ray records once (80 bytes per ray;
+        vir_siddon_ray_records) so fp() and the gather bp() read them instead
         of recomputing each ray's setup per call -- same bits.
         proj: optional per-view projection matrices (nviews, 12) for rays
         already on the device (float32 CUDA tensors of shape (nviews, nu[, nv],
@@ -223,10 +223,14 @@ class SiddonProjector:
         if check:
             self._err.zero_()
         fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._ung
-        _lib.check(_lib.lib().vir_siddon_fp(_lib.ptr(self.src), _lib.ptr(self.trg), self.nrays,
-                                            self.grid, _lib.ptr(vol), _lib.ptr(out), fl,
-                                            _lib.ptr(self._err) if check else None, stream()),
-                   "vir_siddon_fp")
+        err = _lib.ptr(self._err) if check else None
+        if self.rec is not None:
+            _lib.check(_lib.lib().vir_siddon_fp_rec(_lib.ptr(self.rec), self.nrays, self.grid, _lib.ptr(vol),
+                                                    _lib.ptr(out), fl, err, stream()), "vir_siddon_fp_rec")
+        else:
+            _lib.check(_lib.lib().vir_siddon_fp(_lib.ptr(self.src), _lib.ptr(self.trg), self.nrays,
+                                                self.grid, _lib.ptr(vol), _lib.ptr(out), fl, err, stream()),
+                       "vir_siddon_fp")
         if check and int(self._err.item()):
             raise ValueError("invalid entry in coordinates array")
         return out
