@@ -176,8 +176,11 @@ __global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __re
                                                      float* __restrict__ sino, int accumulate, int nut) {
     __shared__ float bnd[2][kDDKMax + 1];  // projected boundary kc + e of the two lines
     __shared__ float val[2][kDDKMax];      // pixel kc + e
-    __shared__ float lsc[kDDLines];        // ray scale of line io
-    __shared__ int2 lband[kDDLines];       // the tile's pixel band of line io (empty: y < x)
+    // per-line tables sized by the launch to max(nx, ny) lines (<= kDDLines):
+    // 8 bytes a line, so small images keep more workgroups resident
+    extern __shared__ float dyn_lds[];
+    float* lsc = dyn_lds;                                                        // ray scale of line io
+    short2* lband = reinterpret_cast<short2*>(dyn_lds + max(G.nx, G.ny));  // the tile's pixel band of line io (empty: y < x)
     const int ut = blockIdx.x % nut, ia = blockIdx.x / nut;
     const int u_lo = ut * 256, u_n = min(256, G.nu - u_lo);
     const int iu = u_lo + (int)threadIdx.x;
@@ -202,10 +205,10 @@ __global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __re
         const float y0 = dd_invert<MODE>(G, V, tt0, oc), y1 = dd_invert<MODE>(G, V, tt1, oc);
         const float t0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd;
         const float t1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd;
-        int2 bandv = make_int2(0, -1);
+        short2 bandv = make_short2(0, -1);  // pixel indices < kDDLines fit 16 bits
         if (t0f < (float)nd + 3.0f && t1f > -3.0f) {
-            bandv.x = (int)fmaxf(floorf(t0f) - 2.0f, 0.0f);
-            bandv.y = (int)fminf(floorf(t1f) + 2.0f, (float)(nd - 1));
+            bandv.x = (short)fmaxf(floorf(t0f) - 2.0f, 0.0f);
+            bandv.y = (short)fminf(floorf(t1f) + 2.0f, (float)(nd - 1));
         }
         lband[io] = bandv;
         lsc[io] = dd_scale<MODE>(G, V, oc);
@@ -241,8 +244,9 @@ __global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __re
         }
     };
     for (int io0 = 0; io0 < no; io0 += 2) {
-        const int2 b0 = lband[io0];
-        const int2 b1 = io0 + 1 < no ? lband[io0 + 1] : make_int2(0, -1);
+        const short2 s0 = lband[io0];
+        const short2 s1 = io0 + 1 < no ? lband[io0 + 1] : make_short2(0, -1);
+        const int2 b0 = make_int2(s0.x, s0.y), b1 = make_int2(s1.x, s1.y);
         const int n0 = b0.y - b0.x + 1, n1 = b1.y - b1.x + 1;
         if (n0 <= 0 && n1 <= 0) continue;
         if (n0 <= kDDKMax && n1 <= kDDKMax) {
@@ -452,11 +456,12 @@ int vir_dd2d_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, c
     if (!(flags & VIR_PER_OUTPUT) && std::max(G.nx, G.ny) <= kDDLines) {
         const int nut = (G.nu + 255) / 256;
         dim3 tg((unsigned)((int64_t)G.na * nut));
+        const size_t tables = (size_t)std::max(G.nx, G.ny) * (sizeof(float) + sizeof(short2));
         if (geom->mode == DD_PAR)
-            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_PAR>, tg, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, nut);
+            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_PAR>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut);
         else if (geom->mode == DD_FAN_FLAT)
-            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_FLAT>, tg, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, nut);
-        else hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_ARC>, tg, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, nut);
+            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_FLAT>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut);
+        else hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_ARC>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut);
         return check_hip(hipGetLastError(), "vir_dd2d_fp launch");
     }
     if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_fp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
