@@ -194,7 +194,7 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
 // footprint apart, so the lanes of a wave never update the same voxel in the
 // same colour phase, and consecutive phases of one wave are ordered by the
 // in-order LDS pipeline -- no atomics and no workgroup barriers.
-template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, bool P2, bool ISO, bool UNG>
+template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, int NS, bool P2, bool ISO, bool UNG>
 __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WPE))) void k_siddon_bp_wave(GridK g, const float* __restrict__ src,
                                                       const float* __restrict__ trg,
                                                       const float* __restrict__ rec, int nviews, int nu,
@@ -203,10 +203,17 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                                                       float* __restrict__ out, int views_per_chunk,
                                                       int nbx, int nby, int nbz, int64_t out_stride) {
     constexpr int kVox = SBX * SBY * SBZ;
-    __shared__ float acc_all[WX * WY * kVox];
+    // NS views at a time per wave: lanes [slot*LPQ, (slot+1)*LPQ) work on view
+    // va0 + slot and accumulate into accumulator copy `slot` of the
+    // sub-brick (copies summed in slot order at the end), so rays of
+    // different views never meet in one accumulator
+    constexpr int LPQ = 64 / NS;
+    __shared__ float acc_all[WX * WY * NS * kVox];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    float* acc = acc_all + wave * kVox;
+    const int slot = lane / LPQ, lq = lane % LPQ;
+    float* acc_w = acc_all + wave * NS * kVox;
+    float* acc = acc_w + slot * kVox;
 
     // nbx/nby/nbz count workgroup regions (WX*SBX x WY*SBY x SBZ)
     const int region = (int)xcd_remap(blockIdx.x, gridDim.x);
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
     const int b1[3] = {min(b0[0] + SBX, g.n[0]), min(b0[1] + SBY, g.n[1]), min(b0[2] + SBZ, g.n[2])};
     const bool live = b0[0] < g.n[0] && b0[1] < g.n[1];
 
-    for (int i = lane; i < kVox; i += 64) acc[i] = 0.0f;
+    for (int i = lane; i < NS * kVox; i += 64) acc_w[i] = 0.0f;
 
     // guard box: the sub-brick grown by kGuard voxels.  An entry whose
     // computed index falls in the sub-brick has its true midpoint within
@@ -233,16 +240,9 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
     const int a_begin = blockIdx.y * views_per_chunk;
     const int a_end = live ? min(nviews, a_begin + views_per_chunk) : a_begin;
 
-    for (int va = a_begin; va < a_end; ++va) {
+    // footprint of view va on the detector, computed by the whole wave
+    auto view_foot = [&](int va) -> ViewFoot {
         const float* P = proj + 12 * va;
-        // this view's rays: ray (va, iu, iv) at in-view offset iu*nv + iv
-        // (the host checks nu nv < 2^30, so 24-bit factors suffice and
-        // 3 (iu nv + iv) fits 32 bits)
-        const int64_t vbase = (int64_t)va * nu * nv;
-        const float* __restrict__ sino_v = sino + vbase;
-        const float* __restrict__ src_v = rec ? nullptr : src + 3 * vbase;
-        const float* __restrict__ trg_v = rec ? nullptr : trg + 3 * vbase;
-        const float* __restrict__ rec_v = rec ? rec + kRayRec * vbase : nullptr;
         // lanes: the 4x4x4 lattice of corner-voxel corners (planes b0, b0+1,
         // b1-1, b1 per axis); lanes 0..7 also project the guard-box corners
         int li = (lane >> 4) & 3, lj = (lane >> 2) & 3, lk = lane & 3;
@@ -315,6 +315,29 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
             f.su = min(f.su, max(f.iu1 - f.iu0 + 1, 1));
             f.sv = min(f.sv, max(f.iv1 - f.iv0 + 1, 1));
         }
+        return f;
+    };
+
+    for (int va0 = a_begin; va0 < a_end; va0 += NS) {
+        // each lane keeps the footprint of its slot's view; slots past the
+        // chunk's last view get an empty one
+        ViewFoot f;
+        f.iu0 = 0; f.iu1 = -1; f.iv0 = 0; f.iv1 = -1; f.su = 1; f.sv = 1;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (va0 + s >= a_end) break;
+            const ViewFoot fs = view_foot(va0 + s);
+            if (s == slot) f = fs;
+        }
+        const int va = min(va0 + slot, a_end - 1);
+        // this view's rays: ray (va, iu, iv) at in-view offset iu*nv + iv
+        // (the host checks nu nv < 2^30, so 24-bit factors suffice and
+        // 3 (iu nv + iv) fits 32 bits)
+        const int64_t vbase = (int64_t)va * nu * nv;
+        const float* __restrict__ sino_v = sino + vbase;
+        const float* __restrict__ src_v = rec ? nullptr : src + 3 * vbase;
+        const float* __restrict__ trg_v = rec ? nullptr : trg + 3 * vbase;
+        const float* __restrict__ rec_v = rec ? rec + kRayRec * vbase : nullptr;
         const int nU = f.iu1 - f.iu0 + 1, nV = f.iv1 - f.iv0 + 1;
         if (nU <= 0 || nV <= 0) continue;
         for (int cu = 0; cu < f.su; ++cu) {
@@ -329,7 +352,7 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                 // less than (q + 0.5)/mV 2^-23 < 0.5/mV while q < 2^22
                 const bool fq = M < (1 << 22);
                 const float inv_mV = 1.0f / (float)mV;
-                for (int q = lane; q < M; q += 64) {
+                for (int q = lq; q < M; q += LPQ) {
                     int qu, qv;
                     if (fq) {
                         qu = (int)(((float)q + 0.5f) * inv_mV);
@@ -403,7 +426,11 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
     for (int i = lane; i < ex * ey * ez; i += 64) {
         int lz = i % ez, ly = (i / ez) % ey, lx = i / (ez * ey);
         int64_t gi = ((int64_t)(b0[0] + lx) * g.n[1] + (b0[1] + ly)) * g.n[2] + (b0[2] + lz);
-        dst[gi] = acc[(lx * SBY + ly) * SBZ + lz];
+        const int li = (lx * SBY + ly) * SBZ + lz;
+        float sum = acc_w[li];
+#pragma unroll
+        for (int s = 1; s < NS; ++s) sum += acc_w[s * kVox + li];
+        dst[gi] = sum;
     }
 }
 
@@ -425,18 +452,22 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int6
 
 // Sub-brick shapes (one per wave, four waves per workgroup): 3D grids use
 // 16x16x8 voxels (8 KiB of f32 accumulators per wave), flat (nZ == 1) grids
-// 16x16x1.  Views are split into chunks until there are enough workgroups to
-// fill the chip; chunk partial volumes are summed in a fixed order.
+// 16x16x1 with four accumulator copies (four views per pass, 4 KiB).  Views
+// are split into chunks until there are enough workgroups to fill the chip;
+// chunk partial volumes are summed in a fixed order.
 // Sub-brick shapes of the backprojector: per-wave brick SBX x SBY x SBZ,
-// WX x WY waves per workgroup, WPE waves per SIMD the register budget allows.
+// WX x WY waves per workgroup, WPE waves per SIMD the register budget allows,
+// NS views per wave pass.
 // 16 x 16 x 8 with 2 x 2 waves was the fastest of the shapes measured for
 // the 256^3 cone-beam case (8^3 to 32 x 32 x 8, one to four waves).
-template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE>
+template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, int NS>
 struct BpShape {
     static constexpr int sbx = SBX, sby = SBY, sbz = SBZ, wx = WX, wy = WY;
 };
-using BpShape3 = BpShape<16, 16, 8, 2, 2, 4>;
-using BpShape2 = BpShape<16, 16, 1, 2, 2, 4>;
+// 2-D sub-bricks see about a dozen rays of one view per colour phase, so a
+// wave works on four views at once, each into its own accumulator copy
+using BpShape3 = BpShape<16, 16, 8, 2, 2, 4, 1>;
+using BpShape2 = BpShape<16, 16, 1, 2, 2, 4, 4>;
 
 struct BpPlan {
     int kind;  // 0: 3D sub-bricks, 1: 2D sub-bricks
@@ -636,10 +667,10 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
         else VIR_BP_LAUNCH1(SHAPE, false);                                                                       \
     } while (0)
 #define VIR_BP_ARGS g, src, trg, rec, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
-#define BP3_ARGS 16, 16, 8, 2, 2, 4
+#define BP3_ARGS 16, 16, 8, 2, 2, 4, 1
 #define BP3_WX BpShape3::wx
 #define BP3_WY BpShape3::wy
-#define BP2_ARGS 16, 16, 1, 2, 2, 4
+#define BP2_ARGS 16, 16, 1, 2, 2, 4, 4
 #define BP2_WX 2
 #define BP2_WY 2
     if (p.kind == 0) VIR_BP_LAUNCH(BP3);
