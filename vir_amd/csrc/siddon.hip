@@ -451,22 +451,24 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int6
 }
 
 // Sub-brick shapes (one per wave, four waves per workgroup): 3D grids use
-// 16x16x8 voxels (8 KiB of f32 accumulators per wave), flat (nZ == 1) grids
-// 16x16x1 with four accumulator copies (four views per pass, 4 KiB).  Views
-// are split into chunks until there are enough workgroups to fill the chip;
-// chunk partial volumes are summed in a fixed order.
+// 16x8x8 voxels with two accumulator copies (two views per pass, 8 KiB of
+// f32 accumulators per wave), flat (nZ == 1) grids 16x16x1 with four (four
+// views per pass, 4 KiB).  Views are split into chunks until there are
+// enough workgroups to fill the chip; chunk partial volumes are summed in a
+// fixed order.
 // Sub-brick shapes of the backprojector: per-wave brick SBX x SBY x SBZ,
 // WX x WY waves per workgroup, WPE waves per SIMD the register budget allows,
-// NS views per wave pass.
-// 16 x 16 x 8 with 2 x 2 waves was the fastest of the shapes measured for
-// the 256^3 cone-beam case (8^3 to 32 x 32 x 8, one to four waves).
+// NS views per wave pass.  For the 256^3 cone-beam case 16 x 8 x 8 with two
+// views per pass and 2 x 2 waves was the fastest measured (one view per
+// pass: 8^3 to 32 x 32 x 8; two or four views: 8^3 to 32 x 16 x 2 within
+// 8 KiB per wave; 1 x 4 and 4 x 1 waves).
 template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, int NS>
 struct BpShape {
     static constexpr int sbx = SBX, sby = SBY, sbz = SBZ, wx = WX, wy = WY;
 };
 // 2-D sub-bricks see about a dozen rays of one view per colour phase, so a
 // wave works on four views at once, each into its own accumulator copy
-using BpShape3 = BpShape<16, 16, 8, 2, 2, 4, 1>;
+using BpShape3 = BpShape<16, 8, 8, 2, 2, 4, 2>;
 using BpShape2 = BpShape<16, 16, 1, 2, 2, 4, 4>;
 
 struct BpPlan {
@@ -667,7 +669,7 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
         else VIR_BP_LAUNCH1(SHAPE, false);                                                                       \
     } while (0)
 #define VIR_BP_ARGS g, src, trg, rec, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
-#define BP3_ARGS 16, 16, 8, 2, 2, 4, 1
+#define BP3_ARGS 16, 8, 8, 2, 2, 4, 2
 #define BP3_WX BpShape3::wx
 #define BP3_WY BpShape3::wy
 #define BP2_ARGS 16, 16, 1, 2, 2, 4, 4
