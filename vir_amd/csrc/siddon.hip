@@ -452,8 +452,8 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int6
 
 // Sub-brick shapes (one per wave, four waves per workgroup): 3D grids use
 // 16x8x8 voxels with two accumulator copies (two views per pass, 8 KiB of
-// f32 accumulators per wave), flat (nZ == 1) grids 16x16x1 with four (four
-// views per pass, 4 KiB).  Views are split into chunks until there are
+// f32 accumulators per wave), flat (nZ == 1) grids 32x16x1 with four (four
+// views per pass, 8 KiB).  Views are split into chunks until there are
 // enough workgroups to fill the chip; chunk partial volumes are summed in a
 // fixed order.
 // Sub-brick shapes of the backprojector: per-wave brick SBX x SBY x SBZ,
@@ -461,15 +461,16 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int6
 // NS views per wave pass.  For the 256^3 cone-beam case 16 x 8 x 8 with two
 // views per pass and 2 x 2 waves was the fastest measured (one view per
 // pass: 8^3 to 32 x 32 x 8; two or four views: 8^3 to 32 x 16 x 2 within
-// 8 KiB per wave; 1 x 4 and 4 x 1 waves).
+// 8 KiB per wave; 1 x 4 and 4 x 1 waves); for 2-D grids 32 x 16 with four
+// views per pass (16 x 16 to 64 x 16, two to eight views, 1024^2 parallel).
 template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, int NS>
 struct BpShape {
     static constexpr int sbx = SBX, sby = SBY, sbz = SBZ, wx = WX, wy = WY;
 };
-// 2-D sub-bricks see about a dozen rays of one view per colour phase, so a
-// wave works on four views at once, each into its own accumulator copy
+// 2-D sub-bricks see few rays of one view per colour phase, so a wave works
+// on four views at once, each into its own accumulator copy
 using BpShape3 = BpShape<16, 8, 8, 2, 2, 4, 2>;
-using BpShape2 = BpShape<16, 16, 1, 2, 2, 4, 4>;
+using BpShape2 = BpShape<32, 16, 1, 2, 2, 4, 4>;
 
 struct BpPlan {
     int kind;  // 0: 3D sub-bricks, 1: 2D sub-bricks
@@ -672,7 +673,7 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
 #define BP3_ARGS 16, 8, 8, 2, 2, 4, 2
 #define BP3_WX BpShape3::wx
 #define BP3_WY BpShape3::wy
-#define BP2_ARGS 16, 16, 1, 2, 2, 4, 4
+#define BP2_ARGS 32, 16, 1, 2, 2, 4, 4
 #define BP2_WX 2
 #define BP2_WY 2
     if (p.kind == 0) VIR_BP_LAUNCH(BP3);
