@@ -277,6 +277,7 @@ constexpr int kKMax = 128;       // pixels of one line handled at once
 constexpr int kWMax = 12;        // pixels whose footprint meets one bin (checked on the host)
 constexpr int kWSlot = kWMax;    // footprint list stride per bin (LDS: 4 workgroups per CU)
 constexpr int kWPer = 3;         // candidate pixels per lane: 4 lanes x 3 cover kWMax
+constexpr int kStageRows = 3;    // pixel rows whose volume loads are in flight together
 
 __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
                                                         const float* __restrict__ cs,
@@ -372,21 +373,33 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
         for (int kc = k0t; kc <= k1t; kc += kKMax) {
             const int nk = min(kKMax, k1t - kc + 1);
             __syncthreads();  // the previous chunk's reads are done
-            // P rows of this wave's pixels, straight from the volume
-            for (int kk = wave; kk < nk; kk += 4) {
-                const int k = kc + kk;
-                const int ix = xd ? k : io, iy = xd ? io : k;
-                const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
-                float xs[kSlabs];
+            // P rows of this wave's pixels, straight from the volume: the
+            // loads of kStageRows rows are in flight together, the first
+            // group's while the footprint lists below are built
+            float xs[kStageRows][kSlabs];
+            auto stage_load = [&](int kk0) {
 #pragma unroll
-                for (int q = 0; q < kSlabs; ++q)
-                    if (q < nzs) xs[q] = col[sz[q]];
-                float sv_ = 0.0f;
+                for (int r = 0; r < kStageRows; ++r) {
+                    const int kk = kk0 + 4 * r;
+                    const int k = kc + min(kk, nk - 1);
+                    const int ix = xd ? k : io, iy = xd ? io : k;
+                    const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
 #pragma unroll
-                for (int q = 0; q < kSlabs; ++q)
-                    if (q < nzs) sv_ += xs[q] * swt[q];
-                P[kk * kTV + lane] = sv_;
-            }
+                    for (int q = 0; q < kSlabs; ++q) xs[r][q] = (q < nzs && kk < nk) ? col[sz[q]] : 0.0f;
+                }
+            };
+            auto stage_store = [&](int kk0) {
+#pragma unroll
+                for (int r = 0; r < kStageRows; ++r) {
+                    const int kk = kk0 + 4 * r;
+                    float sv_ = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < kSlabs; ++q)
+                        if (q < nzs) sv_ += xs[r][q] * swt[q];
+                    if (kk < nk) P[kk * kTV + lane] = sv_;
+                }
+            };
+            stage_load(wave);
             // footprint lists of this wave's bins over the chunk, in pixel
             // order: lane group grp tests candidates ka + 3 grp .. ka + 3 grp + 2
             {
@@ -419,6 +432,10 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
                     }
                 }
                 if (grp == 0) wcnt[tu_w] = ubin ? total : 0;
+            }
+            for (int kk0 = wave; kk0 < nk; kk0 += 4 * kStageRows) {
+                if (kk0 != wave) stage_load(kk0);
+                stage_store(kk0);
             }
             __syncthreads();
             if (mrow) {
