@@ -344,6 +344,15 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
     // the tile's bin edges (footprint margin included)
     const float ut0 = sf_det(G.nu, G.du, G.su, u_lo) - G.wmax;
     const float ut1 = sf_det(G.nu, G.du, G.su, u_lo + u_n) + G.wmax;
+    // sf_invert<SF_PAR3>(u, oc) = (u - po oc) / pd as u/pd - oc po/pd: one
+    // FMA per line.  The inverse only bounds pixel ranges that are widened
+    // by a pixel or more (|pd| >= 1/sqrt 2, rounding moves it by ~1e-4
+    // pixel), and pixels of zero overlap add nothing, so the sums are those
+    // of k_sf_fp.
+    const float pd = xd ? -s : c, po = xd ? c : -s;
+    const float inv_pd = 1.0f / pd, lin_b = po * inv_pd;
+    const float at0 = ut0 * inv_pd, at1 = ut1 * inv_pd;
+    const float al = (ul - G.wmax) * inv_pd, ar = (ur + G.wmax) * inv_pd;
 
     double acc[kTU / 4];
 #pragma unroll
@@ -351,8 +360,8 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
 
     for (int io = 0; io < no; ++io) {
         const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
-        const float x0 = sf_invert<SF_PAR3>(G, xd, c, s, ut0, oc);
-        const float x1 = sf_invert<SF_PAR3>(G, xd, c, s, ut1, oc);
+        const float x0 = __builtin_fmaf(-oc, lin_b, at0);
+        const float x1 = __builtin_fmaf(-oc, lin_b, at1);
         const float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
         const float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
         if (!(k0f < (float)nd + 1.0f) || !(k1f > -2.0f)) continue;
@@ -361,8 +370,8 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
         // this lane's bin: its candidate pixel range (as k_sf_fp)
         int kb0 = 0, kb1 = -1;
         if (ubin) {
-            const float y0 = sf_invert<SF_PAR3>(G, xd, c, s, ul - G.wmax, oc);
-            const float y1 = sf_invert<SF_PAR3>(G, xd, c, s, ur + G.wmax, oc);
+            const float y0 = __builtin_fmaf(-oc, lin_b, al);
+            const float y1 = __builtin_fmaf(-oc, lin_b, ar);
             const float b0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
             const float b1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
             if (b0f < (float)nd + 1.0f && b1f > -2.0f) {
@@ -482,10 +491,13 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
 // (lane = view) and broadcast with readlane in the view loop, which then only
 // reads the sinogram rows each slab meets (consecutive lanes, consecutive
 // rows) and accumulates.  Weights, bin and view order and arithmetic are
-// those of k_sf_bp, so both give the same bits.
+// those of k_sf_bp, so both give the same bits.  The kernel waits on memory:
+// the rows of a view's first kJPre bins are loaded together, and 64 VGPRs
+// keep eight waves per SIMD.
 constexpr int kJMax = 8;  // bins a pixel footprint meets (checked on the host)
+constexpr int kJPre = 2;  // bins whose sinogram rows are loaded together
 
-__global__ __launch_bounds__(256) void k_sf_bp_par3_col(SFGeo G, const float* __restrict__ ang,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_sf_bp_par3_col(SFGeo G, const float* __restrict__ ang,
                                                        const float* __restrict__ cs,
                                                        const float* __restrict__ sn,
                                                        const float* __restrict__ sino, float* __restrict__ img,
@@ -527,6 +539,12 @@ __global__ __launch_bounds__(256) void k_sf_bp_par3_col(SFGeo G, const float* __
         }
     }
     if (!zin || SW.count <= 0) nzr = 0;
+
+    // rows any lane of the wave needs per bin (wave-uniform)
+    int nzr_max = nzr;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) nzr_max = max(nzr_max, __shfl_xor(nzr_max, m));
+    const bool pre = __builtin_amdgcn_readfirstlane(nzr_max) <= 2;
 
     const float u0 = sf_det(G.nu, G.du, G.su, 0);
     const float inv_du = 1.0f / G.du;
@@ -570,16 +588,36 @@ __global__ __launch_bounds__(256) void k_sf_bp_par3_col(SFGeo G, const float* __
         for (int v = 0; v < nvw; ++v) {
             const int c = __builtin_amdgcn_readlane(cnt, v);
             const float* plane = sb + (int64_t)(va + v) * G.nu * G.nv;
+            // the rows of the first kJPre bins, two per slab, are loaded
+            // together before any is used, by every lane and for every one
+            // of these bins: bins past the footprint's count read bin jl = 0,
+            // rows past the slab's count row rq = 0, both valid, and the sums
+            // below skip them
+            float xv[kJPre][2];
+            if (pre) {
+#pragma unroll
+                for (int s = 0; s < kJPre; ++s) {
+                    const float* row = plane + (int64_t)__builtin_amdgcn_readlane(jl[s], v) * G.nv;
+                    xv[s][0] = row[rq[0]];
+                    xv[s][1] = row[rq[1]];
+                }
+            }
 #pragma unroll
             for (int s = 0; s < kJMax; ++s) {
                 if (s >= c) break;
-                const int j = __builtin_amdgcn_readlane(jl[s], v);
                 const float wu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wl[s]), v));
-                const float* row = plane + (int64_t)j * G.nv;
                 float sv_ = 0.0f;
+                if (s < kJPre && pre) {
+                    const int sp = s < kJPre ? s : 0;
+                    sv_ = 0 < nzr ? sv_ + xv[sp][0] * rw[0] : sv_;
+                    sv_ = 1 < nzr ? sv_ + xv[sp][1] * rw[1] : sv_;
+                } else {
+                    const int j = __builtin_amdgcn_readlane(jl[s], v);
+                    const float* row = plane + (int64_t)j * G.nv;
 #pragma unroll
-                for (int q = 0; q < kSlabs; ++q)
-                    if (q < nzr) sv_ += row[rq[q]] * rw[q];
+                    for (int q = 0; q < kSlabs; ++q)
+                        if (q < nzr) sv_ += row[rq[q]] * rw[q];
+                }
                 acc += (double)(wu * sv_);
             }
         }
