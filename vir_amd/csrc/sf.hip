@@ -334,6 +334,12 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
         }
     }
     if (!mrow) nzs = 0;
+    // slabs any lane of the wave needs (wave-uniform): the staging loads
+    // are bounded by it, not masked per lane
+    int nzs_max = nzs;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) nzs_max = max(nzs_max, __shfl_xor(nzs_max, m));
+    const int nzs_u = __builtin_amdgcn_readfirstlane(nzs_max);
 
     // footprint-list lanes: bin tu_w (this wave's 16 bins), candidate group grp
     const int tu_w = wave * (kTU / 4) + (lane & 15), grp = lane >> 4;
@@ -384,7 +390,9 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
             __syncthreads();  // the previous chunk's reads are done
             // P rows of this wave's pixels, straight from the volume: the
             // loads of kStageRows rows are in flight together, the first
-            // group's while the footprint lists below are built
+            // group's while the footprint lists below are built.  Every lane
+            // loads: sz[q] = 0 where q >= nzs and the row index is clamped
+            // to the chunk, both valid, and the sums skip those values.
             float xs[kStageRows][kSlabs];
             auto stage_load = [&](int kk0) {
 #pragma unroll
@@ -394,7 +402,7 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
                     const int ix = xd ? k : io, iy = xd ? io : k;
                     const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
 #pragma unroll
-                    for (int q = 0; q < kSlabs; ++q) xs[r][q] = (q < nzs && kk < nk) ? col[sz[q]] : 0.0f;
+                    for (int q = 0; q < kSlabs; ++q) xs[r][q] = q < nzs_u ? col[sz[q]] : 0.0f;
                 }
             };
             auto stage_store = [&](int kk0) {
