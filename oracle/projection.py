@@ -39,6 +39,23 @@ def bp64(sino, ind, ln, off, nvox):
     return np.bincount(np.asarray(ind, np.int64), weights=w, minlength=nvox)
 
 
+def sd_b_proj_numpy(sino, ind, ln, off, nPixels):
+    """The reference's numpy backprojector (projection/proj.py:119-150,
+    ravel=True branch :133-136) over a CSR matrix: float32 volume, rays in
+    order, `phantom_flat[idx] += len * sino[r]` with numpy's buffered
+    fancy-index `+=`, so a voxel repeated within a ray keeps one update."""
+    phantom = np.zeros(tuple(nPixels), dtype=np.float32)
+    flat = phantom.ravel()
+    s = np.asarray(sino, np.float32).ravel()
+    ind = np.asarray(ind)
+    ln = np.asarray(ln, np.float32)
+    for r in range(len(off) - 1):
+        a, b = int(off[r]), int(off[r + 1])
+        if b > a:
+            flat[ind[a:b]] += ln[a:b] * s[r]
+    return phantom
+
+
 class _RaveledRay(ctypes.Structure):
     # projection_c.c:61-65
     _fields_ = [("n", ctypes.c_int), ("R", ctypes.POINTER(ctypes.c_int)),

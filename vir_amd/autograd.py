@@ -17,7 +17,7 @@ the same data term and regulariser.
 import torch
 
 from . import _lib
-from .device import stream
+from .device import require_cuda_f32, stream
 
 TV_WORK = 2048  # doubles, VIR_TV_WORK
 
@@ -44,12 +44,15 @@ def project(x, P):
 
 
 class LeastSquares(torch.autograd.Function):
-    """0.5 * ||a - b||^2 as a float64 scalar; dL/da = a - b."""
+    """0.5 * ||a - b||^2 as a float64 scalar; dL/da = a - b, dL/db = b - a."""
 
     @staticmethod
     def forward(ctx, a, b):
+        require_cuda_f32(a, b, what="least_squares inputs")
+        if a.device != b.device:
+            raise ValueError("least_squares inputs are on different devices")
         a = a.detach().contiguous()
-        b = b.detach().to(device=a.device, dtype=torch.float32).contiguous()
+        b = b.detach().contiguous()
         if a.numel() != b.numel():
             raise ValueError("a and b differ in size")
         r = torch.empty_like(a)
@@ -63,7 +66,9 @@ class LeastSquares(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gv):
         (r,) = ctx.saved_tensors
-        return r * gv.to(r.dtype), None
+        ga = r * gv.to(r.dtype)
+        gb = -ga if ctx.needs_input_grad[1] else None
+        return (ga if ctx.needs_input_grad[0] else None), gb
 
 
 def least_squares(a, b):
@@ -91,9 +96,8 @@ class TotalVariation(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, x, eps, ndim):
+        require_cuda_f32(x, what="total_variation input")
         x = x.detach().contiguous()
-        if x.dtype != torch.float32:
-            raise TypeError("total_variation expects float32")
         nb, nx, ny, nz = _tv_shape(x, ndim)
         grad = torch.empty_like(x)
         val = torch.empty((), dtype=torch.float64, device=x.device)

@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .device import as_device_f32, require_gpu, stream
+from .device import as_device_f32, out_tensor, require_gpu, stream
 
 _MODES = {"flat": 1, "arc": 2}
 
@@ -175,8 +175,7 @@ class FBP:
 
     def __call__(self, sino, out=None):
         q = self.filtered(sino)
-        if out is None:
-            out = torch.empty(self.img_shape, dtype=torch.float32, device=q.device)
+        out = out_tensor(out, self.img_shape, q.device)
         _lib.check(_lib.lib().vir_fbp_bp(self.g, _lib.ptr(self.cos), _lib.ptr(self.sin), _lib.ptr(q),
                                          _lib.ptr(out), self.scale, 0, stream()), "vir_fbp_bp")
         return out

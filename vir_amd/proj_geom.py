@@ -103,6 +103,42 @@ def check_in_front(mats, nPixels, dPixels, origin, margin=2.0):
     return bool(np.all(w > 1e-6 * np.abs(w).max()))
 
 
+def check_projections(src, trg, mats, tol=0.05):
+    """Raise ValueError unless each view's 3x4 matrix maps its own rays onto
+    their ray indices: the four corner rays and the centre ray of every view
+    (src, trg: (nviews, nu, nv, 3) numpy arrays or device tensors; only
+    those 5 rays per view are read) must project to (iu, iv) within `tol`
+    index units.  The midpoint of a ray is on the ray for a point source and
+    for a parallel bundle alike, so it is the point projected."""
+    nviews, nu, nv = (int(s) for s in trg.shape[:3])
+    pick = sorted({(0, 0), (0, nv - 1), (nu - 1, 0), (nu - 1, nv - 1), (nu // 2, nv // 2)})
+    iu_idx = np.array([p[0] for p in pick])
+    iv_idx = np.array([p[1] for p in pick])
+    if hasattr(trg, "is_cuda"):
+        import torch
+        ti = torch.as_tensor(iu_idx, device=trg.device)
+        tv = torch.as_tensor(iv_idx, device=trg.device)
+        s = src[:, ti, tv].double().cpu().numpy()
+        t = trg[:, ti, tv].double().cpu().numpy()
+    else:
+        s = np.asarray(src, np.float64)[:, iu_idx, iv_idx]
+        t = np.asarray(trg, np.float64)[:, iu_idx, iv_idx]
+    M = np.asarray(mats, np.float64).reshape(nviews, 3, 4)
+    X = np.concatenate([0.5 * (s + t), np.ones(s.shape[:2] + (1,))], axis=2)
+    h = np.einsum("aij,akj->aki", M, X)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pu = h[..., 0] / h[..., 2]
+        pv = h[..., 1] / h[..., 2]
+    err = np.abs(pu - iu_idx[None, :])
+    if nv > 1:
+        err = np.maximum(err, np.abs(pv - iv_idx[None, :]))
+    bad = ~(np.isfinite(err) & (err <= tol) & (h[..., 2] > 0))
+    if bad.any():
+        a = int(np.nonzero(bad.any(axis=1))[0][0])
+        raise ValueError(f"view {a}: the projection matrix does not map the view's rays onto their "
+                         f"detector indices (error {np.nanmax(err[a]):.3g} > {tol})")
+
+
 def view_projections(src, trg, tol=0.05):
     """Per-view 3x4 projection matrices for a structured ray set.
 

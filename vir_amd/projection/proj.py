@@ -60,8 +60,42 @@ def _to_csr(sdlist, nPixels, ravel):
     return ind, ln, off
 
 
-def _device_csr(csr):
+def check_csr(ind, ln, off, nvox):
+    """Reject a stored matrix whose entries the kernels would read out of
+    bounds, before anything is launched: every index in [0, nvox) (the
+    reference's fancy indexing raises IndexError, proj.py:92-93,127-128),
+    off[0] == 0, off non-decreasing, off[-1] == len(ind) == len(ln)
+    (ValueError).  numpy arrays are checked on the host, device tensors with
+    torch reductions on the device."""
+    if isinstance(off, torch.Tensor):
+        nnz, nl, n_off = ind.numel(), ln.numel(), off.numel()
+        first = int(off[0].item()) if n_off else 0
+        last = int(off[-1].item()) if n_off else 0
+        mono = n_off < 2 or bool((off[1:] >= off[:-1]).all().item())
+        lo = int(ind.min().item()) if nnz else 0
+        hi = int(ind.max().item()) if nnz else 0
+    else:
+        off = np.asarray(off)
+        nnz, nl, n_off = np.size(ind), np.size(ln), off.size
+        first = int(off[0]) if n_off else 0
+        last = int(off[-1]) if n_off else 0
+        mono = n_off < 2 or bool(np.all(off[1:] >= off[:-1]))
+        lo = int(np.min(ind)) if nnz else 0
+        hi = int(np.max(ind)) if nnz else 0
+    if n_off < 1:
+        raise ValueError("offsets must hold nrays + 1 >= 1 values")
+    if first != 0 or not mono or last != nnz or nl != nnz:
+        raise ValueError("offsets must start at 0, not decrease and end at the number of entries "
+                         f"(off[0]={first}, off[-1]={last}, {nnz} indices, {nl} lengths)")
+    if nnz and (lo < 0 or hi >= nvox):
+        bad = lo if lo < 0 else hi
+        raise IndexError(f"index {bad} is out of bounds for a volume of size {nvox}")
+
+
+def _device_csr(csr, nvox):
     ind, ln, off = csr
+    if isinstance(off, torch.Tensor):
+        check_csr(ind, ln, off, nvox)
     return (as_device(ind, torch.int32), as_device_f32(ln), as_device(off, torch.int64))
 
 
@@ -80,11 +114,14 @@ def sd_f_proj(phantom, sdlist, ravel=True, flat=True, sino_shape=None, sequentia
     sequential=True sums each ray in entry order in plain float32, the
     rounding of projection_c.c:101-157; the default accumulates in float64 as
     proj_numba.py:17 / proj_cuda.py:22 do."""
-    require_gpu()
     torch_in = isinstance(phantom, torch.Tensor)
     nPixels = tuple(phantom.shape)
     shape = _ray_shape(sdlist, sino_shape)
-    ind, ln, off = _device_csr(_to_csr(sdlist, nPixels, ravel or _is_flat(sdlist)))
+    nvox = int(np.prod(nPixels))
+    csr = _to_csr(sdlist, nPixels, ravel or _is_flat(sdlist))
+    check_csr(*csr, nvox)
+    require_gpu()
+    ind, ln, off = _device_csr(csr, nvox)
     nrays = off.numel() - 1
     vol = as_device_f32(phantom)
     sino = torch.zeros(int(np.prod(shape)), dtype=torch.float32, device=vol.device)
@@ -97,15 +134,50 @@ def sd_f_proj(phantom, sdlist, ravel=True, flat=True, sino_shape=None, sequentia
     return sino if torch_in else sino.cpu().numpy()
 
 
-def sd_b_proj(sino, sdlist, nPixels, ravel=True, sequential=False):
+def last_of_repeats(ind, ln, off):
+    """The stored matrix with every entry whose voxel appears again later in
+    the same ray removed (host arrays).  numpy's buffered fancy-index `+=`
+    (proj.py:134, 139-140) writes each repeated voxel once, from its last
+    entry, so this is the matrix the reference's sd_b_proj applies."""
+    ind = np.asarray(ind)
+    off = np.asarray(off, np.int64)
+    nnz = ind.size
+    if nnz == 0:
+        return ind, np.asarray(ln), off
+    ray = np.repeat(np.arange(off.size - 1), np.diff(off))
+    pos = np.arange(nnz)
+    order = np.lexsort((pos, ind, ray))          # by ray, voxel, then position
+    last = np.ones(nnz, bool)
+    same = (ray[order][1:] == ray[order][:-1]) & (ind[order][1:] == ind[order][:-1])
+    last[order[:-1][same]] = False               # a later entry of the same voxel follows
+    counts = np.bincount(ray[last], minlength=off.size - 1)
+    new_off = np.zeros_like(off)
+    np.cumsum(counts, out=new_off[1:])
+    return ind[last], np.asarray(ln)[last], new_off
+
+
+def sd_b_proj(sino, sdlist, nPixels, ravel=True, sequential=False, compat=False):
     """Backprojection phantom[v] = sum over entries (r, v) of len * sino[r]
     (the exact transpose of sd_f_proj).  sequential=True reproduces the
-    float32 rounding sequence of projection_c.c:176-208."""
-    require_gpu()
+    float32 rounding sequence of projection_c.c:176-208.
+
+    compat=True reproduces the reference's numpy backprojector
+    (proj.py:119-150): a voxel repeated within one ray receives only its last
+    entry's value (fancy-index `+=`).  With sequential=True as well the
+    result is the reference's bit for bit (rays added in order in float32,
+    each product len * sino in float32)."""
     torch_in = isinstance(sino, torch.Tensor)
     nPixels = tuple(int(v) for v in np.atleast_1d(nPixels))
     nvox = int(np.prod(nPixels))
-    ind, ln, off = _device_csr(_to_csr(sdlist, nPixels, ravel or _is_flat(sdlist)))
+    csr = _to_csr(sdlist, nPixels, ravel or _is_flat(sdlist))
+    if compat:
+        csr = last_of_repeats(*(c.cpu().numpy() if isinstance(c, torch.Tensor) else c for c in csr))
+    check_csr(*csr, nvox)
+    if int(np.prod(np.shape(sino))) < len(csr[2]) - 1:
+        # sino_flat[i] of proj.py:139-140 raises for a short sinogram
+        raise IndexError(f"sinogram has {int(np.prod(np.shape(sino)))} values for {len(csr[2]) - 1} rays")
+    require_gpu()
+    ind, ln, off = _device_csr(csr, nvox)
     s = as_device_f32(sino).reshape(-1)
     colptr, row, clen = csr_transpose(ind, ln, off, nvox)
     vol = torch.empty(nvox, dtype=torch.float32, device=s.device)

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .device import as_device, as_device_f32, require_gpu, stream
+from .device import as_device_f32, out_tensor, require_gpu, stream
 
 
 class StoredMatrix:
@@ -28,7 +28,7 @@ class StoredMatrix:
     CSR gather forward, voxel-major transpose gather backward."""
 
     def __init__(self, sdlist, nPixels, ray_shape=None):
-        from .projection.proj import _to_csr, _device_csr, _is_flat
+        from .projection.proj import _to_csr, _device_csr, _is_flat, check_csr
         from .sys_mat.siddon import csr_transpose
         require_gpu()
         self.img_shape = tuple(int(v) for v in np.atleast_1d(nPixels))
@@ -40,16 +40,21 @@ class StoredMatrix:
             ravel = first is None or len(first) == 2
             if ray_shape is None:
                 ray_shape = np.asarray(sdlist, dtype=object).shape
-        self.ind, self.len, self.off = _device_csr(_to_csr(sdlist, self.img_shape, ravel))
+        csr = _to_csr(sdlist, self.img_shape, ravel)
+        check_csr(*csr, nvox)
+        self.ind, self.len, self.off = _device_csr(csr, nvox)
         self.nrays = self.off.numel() - 1
         self.sino_shape = tuple(ray_shape) if ray_shape is not None else (self.nrays,)
+        if int(np.prod(self.sino_shape)) != self.nrays:
+            raise ValueError(f"ray shape {self.sino_shape} does not hold {self.nrays} rays")
         self.nvox = nvox
         self.colptr, self.row, self.clen = csr_transpose(self.ind, self.len, self.off, nvox)
 
     def fp(self, x, out=None, accumulate=False):
         x = as_device_f32(x)
-        if out is None:
-            out = torch.empty(self.sino_shape, dtype=torch.float32, device=x.device)
+        if x.numel() != self.nvox:
+            raise ValueError("volume size does not match the matrix")
+        out = out_tensor(out, self.sino_shape, x.device)
         fl = _lib.VIR_ACCUMULATE if accumulate else 0
         _lib.check(_lib.lib().vir_csr_fp(_lib.ptr(self.ind), _lib.ptr(self.len), _lib.ptr(self.off),
                                          self.nrays, _lib.ptr(x), _lib.ptr(out), fl, stream()),
@@ -58,8 +63,9 @@ class StoredMatrix:
 
     def bp(self, y, out=None, accumulate=False):
         y = as_device_f32(y)
-        if out is None:
-            out = torch.empty(self.img_shape, dtype=torch.float32, device=y.device)
+        if y.numel() != self.nrays:
+            raise ValueError("sinogram size does not match the matrix")
+        out = out_tensor(out, self.img_shape, y.device)
         fl = _lib.VIR_ACCUMULATE if accumulate else 0
         _lib.check(_lib.lib().vir_csc_bp(_lib.ptr(self.colptr), _lib.ptr(self.row), _lib.ptr(self.clen),
                                          self.nvox, _lib.ptr(y), _lib.ptr(out), fl, stream()),

@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ..device import as_device_f32, require_gpu, stream
+from ..device import as_device_f32, out_tensor, require_gpu, stream
 
 _MODES = {"par": 0, "flat": 1, "arc": 2}
 
@@ -97,8 +97,7 @@ class DDProjector:
         img = as_device_f32(img)
         if img.numel() != int(np.prod(self.img_shape)):
             raise ValueError("image size does not match")
-        if out is None:
-            out = torch.empty(self.sino_shape, dtype=torch.float32, device=img.device)
+        out = out_tensor(out, self.sino_shape, img.device)
         self._call("vir_ddcone_fp" if self.geom == "cone" else "vir_dd2d_fp", img, out,
                    self.fp_scale, accumulate)
         return out
@@ -109,8 +108,7 @@ class DDProjector:
         sino = as_device_f32(sino)
         if sino.numel() != int(np.prod(self.sino_shape)):
             raise ValueError("sinogram size does not match")
-        if out is None:
-            out = torch.empty(self.img_shape, dtype=torch.float32, device=sino.device)
+        out = out_tensor(out, self.img_shape, sino.device)
         scale = self.ref_bp_scale if reference_scale else self.fp_scale
         self._call("vir_ddcone_bp" if self.geom == "cone" else "vir_dd2d_bp", sino, out, scale,
                    accumulate)
@@ -213,8 +211,7 @@ class SquareDDProjector:
         img = as_device_f32(img)
         if img.numel() != int(np.prod(self.img_shape)):
             raise ValueError("image size does not match")
-        if out is None:
-            out = torch.empty(self.sino_shape, dtype=torch.float32, device=img.device)
+        out = out_tensor(out, self.sino_shape, img.device)
         fl = _lib.VIR_ACCUMULATE if accumulate else 0
         _lib.check(_lib.lib().vir_ddsq_fp(self.g, _lib.ptr(img), _lib.ptr(out), self.fp_scale, fl,
                                           stream()), "vir_ddsq_fp")
@@ -224,8 +221,7 @@ class SquareDDProjector:
         sino = as_device_f32(sino)
         if sino.numel() != int(np.prod(self.sino_shape)):
             raise ValueError("sinogram size does not match")
-        if out is None:
-            out = torch.empty(self.img_shape, dtype=torch.float32, device=sino.device)
+        out = out_tensor(out, self.img_shape, sino.device)
         fl = _lib.VIR_ACCUMULATE if accumulate else 0
         _lib.check(_lib.lib().vir_ddsq_bp(self.g, _lib.ptr(sino), _lib.ptr(out), 1.0, fl, stream()),
                    "vir_ddsq_bp")

@@ -32,7 +32,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ..device import as_device_f32, require_gpu, stream
+from ..device import as_device_f32, out_tensor, require_gpu, stream
 
 _MODES = {"par": 0, "fan": 1, "cone": 2, "par3d": 3}
 
@@ -87,8 +87,7 @@ class SFProjector:
         img = as_device_f32(img)
         if img.numel() != int(np.prod(self.vol_shape)):
             raise ValueError("image size does not match")
-        if out is None:
-            out = torch.empty(self.sino_shape, dtype=torch.float32, device=img.device)
+        out = out_tensor(out, self.sino_shape, img.device)
         self._call("vir_sf_fp", img, out, self.fp_scale, accumulate)
         return out
 
@@ -98,8 +97,7 @@ class SFProjector:
         sino = as_device_f32(sino)
         if sino.numel() != int(np.prod(self.sino_shape)):
             raise ValueError("sinogram size does not match")
-        if out is None:
-            out = torch.empty(self.vol_shape, dtype=torch.float32, device=sino.device)
+        out = out_tensor(out, self.vol_shape, sino.device)
         scale = self.ref_bp_scale if reference_scale else self.fp_scale
         self._call("vir_sf_bp", sino, out, scale, accumulate)
         return out

@@ -32,7 +32,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ..device import as_device_f32, require_gpu, stream
+from ..device import as_device_f32, out_tensor, require_gpu, stream
 
 _MODES = {"par": 0, "fan": 1, "cone": 2}
 
@@ -80,8 +80,7 @@ class RDProjector:
         img = as_device_f32(img)
         if img.numel() != int(np.prod(self.img_shape)):
             raise ValueError("image size does not match")
-        if out is None:
-            out = torch.empty(self.sino_shape, dtype=torch.float32, device=img.device)
+        out = out_tensor(out, self.sino_shape, img.device)
         self._call("vir_rd_fp", img, out, 1.0, accumulate)
         return out
 
@@ -91,8 +90,7 @@ class RDProjector:
         sino = as_device_f32(sino)
         if sino.numel() != int(np.prod(self.sino_shape)):
             raise ValueError("sinogram size does not match")
-        if out is None:
-            out = torch.empty(self.img_shape, dtype=torch.float32, device=sino.device)
+        out = out_tensor(out, self.img_shape, sino.device)
         self._call("vir_rd_bp", sino, out, self.ref_bp_scale if reference_scale else 1.0,
                    accumulate)
         return out

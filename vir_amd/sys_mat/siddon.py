@@ -15,8 +15,8 @@ import torch
 
 from .. import _lib
 from ..core import np_array
-from ..device import as_device, as_device_f32, require_gpu, stream
-from ..proj_geom import check_in_front, view_projections
+from ..device import as_device, as_device_f32, out_tensor, require_gpu, stream
+from ..proj_geom import check_in_front, check_projections, view_projections
 
 
 def grid_params(nPixels, dPixels, origin):
@@ -157,6 +157,11 @@ class SiddonProjector:
             mats = np.asarray(proj, np.float32).reshape(shape[0], 12)
             if not check_in_front(mats, n, d, o):
                 raise ValueError("a view's source lies inside the volume's guard region")
+            # the backprojector takes its candidate rays and colour strides
+            # from these matrices: they must describe these rays
+            nv_ = shape[2] if len(shape) == 3 else 1
+            check_projections(self.src.reshape(shape[0], shape[1], nv_, 3),
+                              self.trg.reshape(shape[0], shape[1], nv_, 3), mats)
             self.proj = torch.from_numpy(mats).to(self.src.device)
             self.layout = (shape[0], shape[1], shape[2] if len(shape) == 3 else 1)
         else:
@@ -218,8 +223,7 @@ class SiddonProjector:
         vol = as_device_f32(vol)
         if vol.numel() != self.nvox:
             raise ValueError("volume size does not match the grid")
-        if out is None:
-            out = torch.empty(self.ray_shape, dtype=torch.float32, device=vol.device)
+        out = out_tensor(out, self.ray_shape, vol.device)
         if check:
             self._err.zero_()
         fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._ung
@@ -241,8 +245,7 @@ class SiddonProjector:
         sino = as_device_f32(sino)
         if sino.numel() != self.nrays:
             raise ValueError("sinogram size does not match the ray set")
-        if out is None:
-            out = torch.empty(self.nPixels, dtype=torch.float32, device=sino.device)
+        out = out_tensor(out, self.nPixels, sino.device)
         fl = _lib.VIR_ACCUMULATE if accumulate else 0
         if self.proj is not None:
             fl |= self._ung
