@@ -336,3 +336,55 @@ def overlap_brute_2d(nx, ny, ang, nu, du, su, d_pix, geom="par", DSO=None, DSD=N
                     if ov > 0:
                         A[ia, iu, ix, iy] = ov * scl * d_pix / du
     return A.reshape(ang.size * nu, nx * ny)
+
+
+def overlap_sparse_2d(nx, ny, ang, nu, du, su, d_pix, geom="par", DSO=None, DSD=None):
+    """overlap_brute_2d's system matrix (same formula, float64) as a
+    scipy.sparse CSR matrix, vectorised over pixels: each pixel's projected
+    driving-axis segment [lo, hi] against the few bins it meets.  For
+    workload-sized grids (512^2 x 888 bins) where the dense loop is out of
+    reach; pinned against overlap_brute_2d in tests/test_dd_oracle.py."""
+    import scipy.sparse as sp
+    ang = np.asarray(ang, np.float64)
+    trig_c, trig_s = _trig(ang)
+    ub = _ubnd(nu, du, su)
+    xb, yb = _bnd(nx, d_pix), _bnd(ny, d_pix)
+    xc, yc = _cen(xb), _cen(yb)
+    rows, cols, vals = [], [], []
+    pix = np.arange(nx * ny).reshape(nx, ny)
+    for ia in range(ang.size):
+        c, s = trig_c[ia], trig_s[ia]
+        xdrv = abs(s) >= abs(c)
+        if xdrv:
+            pts = [(xb[:-1, None], yc[None, :]), (xb[1:, None], yc[None, :])]
+        else:
+            pts = [(xc[:, None], yb[None, :-1]), (xc[:, None], yb[None, 1:])]
+        if geom == "par":
+            us = [-s * x + c * y for x, y in pts]
+            scl = np.full((nx, ny), 1.0 / (abs(s) if xdrv else abs(c)))
+        else:
+            if geom == "arc":
+                us = [DSD * np.arctan2(-s * x + c * y, DSO - (c * x + s * y)) for x, y in pts]
+            else:
+                us = [DSD * (-s * x + c * y) / (DSO - (c * x + s * y)) for x, y in pts]
+            if xdrv:
+                po, oo = np.broadcast_to(c * yc[None, :], (nx, ny)), np.broadcast_to(s * yc[None, :], (nx, ny))
+            else:
+                po, oo = np.broadcast_to(-s * xc[:, None], (nx, ny)), np.broadcast_to(c * xc[:, None], (nx, ny))
+            rs = abs(s) if xdrv else abs(c)
+            scl = np.sqrt((DSO - oo) ** 2 + po ** 2) / ((DSO - oo) * rs)
+        u0, u1 = np.broadcast_arrays(*us)
+        lo, hi = np.minimum(u0, u1), np.maximum(u0, u1)
+        first = np.clip(np.searchsorted(ub, lo, side="right") - 1, 0, nu - 1)
+        span = int(np.max(np.ceil((hi - lo) / du))) + 2
+        for k in range(span):
+            iu = first + k
+            ok = iu < nu
+            iuc = np.minimum(iu, nu - 1)
+            ov = np.minimum(hi, ub[iuc + 1]) - np.maximum(lo, ub[iuc])
+            ok &= ov > 0
+            rows.append((ia * nu + iuc)[ok])
+            cols.append(pix[ok])
+            vals.append((ov * scl * d_pix / du)[ok])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(ang.size * nu, nx * ny))

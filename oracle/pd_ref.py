@@ -146,3 +146,44 @@ def fan_fp_loop(img, ang_arr, nu, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, stale_oy=
                 sino[ia] += img[ix, iy] * amp * np.where(o > 0, o, 0.0) / du
                 oy_l = oy[iy + 1]
     return sino * d_pix
+
+
+def par3d_ops(img_shape, ang_arr, nu, nv, du=1.0, dv=1.0, su=0.0, sv=0.0, d_pix=1.0):
+    """Forward and adjoint of matrix("par3d", ...) without the dense matrix,
+    for workload-sized volumes: per view, the 2-D parallel footprint
+    weights Wu (nu, nx*ny) as a sparse matrix and the slab/row overlaps
+    Wv (nz, nv), so that  W img = Wu @ (img(nx*ny, nz) @ Wv)  per view.
+    Returns (fp(img) -> (na, nu, nv) = d_pix W img, wt(sino) -> W^T sino).
+    Pinned against matrix("par3d") in tests/test_pd_oracle.py."""
+    import scipy.sparse as sp
+    nx, ny, nz = img_shape
+    ub = det_bnd(nu, du, su)
+    vb = det_bnd(nv, dv, sv)
+    zb = _bnd(nz, d_pix)
+    Wv = _overlap(zb[:-1], zb[1:], vb) / dv                     # (nz, nv)
+    mats = []
+    for a in np.asarray(ang_arr, np.float64).reshape(-1):
+        u0, u1, amp, _ = footprints("par", nx, ny, d_pix, a)
+        u0, u1, amp = u0.reshape(-1), u1.reshape(-1), amp.reshape(-1)
+        first = np.clip(np.searchsorted(ub, u0, side="right") - 1, 0, nu - 1)
+        span = int(np.max(np.ceil((u1 - u0) / du))) + 2
+        rows, cols, vals = [], [], []
+        for k in range(span):
+            iu = np.minimum(first + k, nu - 1)
+            ov = np.minimum(u1, ub[iu + 1]) - np.maximum(u0, ub[iu])
+            ok = (first + k < nu) & (ov > 0)
+            rows.append(iu[ok])
+            cols.append(np.nonzero(ok)[0])
+            vals.append((ov / du * amp)[ok])
+        mats.append(sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                                  shape=(nu, nx * ny)))
+
+    def fp_(img):
+        x = np.asarray(img, np.float64).reshape(nx * ny, nz) @ Wv
+        return d_pix * np.stack([M @ x for M in mats])
+
+    def wt(sino):
+        y = np.asarray(sino, np.float64).reshape(len(mats), nu, nv)
+        out = sum(M.T @ (y[i] @ Wv.T) for i, M in enumerate(mats))
+        return np.asarray(out).reshape(nx, ny, nz)
+    return fp_, wt

@@ -39,6 +39,55 @@ def bp64(sino, ind, ln, off, nvox):
     return np.bincount(np.asarray(ind, np.int64), weights=w, minlength=nvox)
 
 
+class CsrOperator:
+    """A stored CSR matrix (int32 index, float32 length, int64 offsets) as a
+    float64 operator with `A @ x`, `A.T @ y` and `A.shape`, the products
+    computed by the C oracle on `nthreads` OpenMP threads
+    (oracle_csr_fp64 / oracle_csr_bp64).  Lets oracle/sirt_ref.sirt run at
+    workload size.  Pinned against scipy.sparse in tests/test_oracle_siddon.py."""
+
+    def __init__(self, ind, ln, off, ncols, nthreads=None, _t=False):
+        from . import siddon as osd
+        self.ind = np.ascontiguousarray(ind, np.int32)
+        self.ln = np.ascontiguousarray(ln, np.float32)
+        self.off = np.ascontiguousarray(off, np.int64)
+        self.nrows = self.off.size - 1
+        self.ncols = int(ncols)
+        self.nthreads = int(nthreads or max(1, min(16, len(os.sched_getaffinity(0)))))
+        self._t = _t
+        self._lib = osd._load()
+        P = ctypes.c_void_p
+        self._lib.oracle_csr_fp64.restype = None
+        self._lib.oracle_csr_fp64.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_int]
+        self._lib.oracle_csr_bp64.restype = ctypes.c_int
+        self._lib.oracle_csr_bp64.argtypes = [ctypes.c_int64, P, P, P, ctypes.c_int64, P, P, ctypes.c_int]
+
+    @property
+    def shape(self):
+        return (self.ncols, self.nrows) if self._t else (self.nrows, self.ncols)
+
+    @property
+    def T(self):
+        t = CsrOperator.__new__(CsrOperator)
+        t.__dict__.update(self.__dict__)
+        t._t = not self._t
+        return t
+
+    def __matmul__(self, v):
+        v = np.ascontiguousarray(np.asarray(v, np.float64).reshape(-1))
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        if not self._t:
+            out = np.empty(self.nrows)
+            self._lib.oracle_csr_fp64(self.nrows, p(self.ind), p(self.ln), p(self.off), p(v), p(out),
+                                      self.nthreads)
+        else:
+            out = np.empty(self.ncols)
+            if self._lib.oracle_csr_bp64(self.nrows, p(self.ind), p(self.ln), p(self.off), self.ncols,
+                                         p(v), p(out), self.nthreads):
+                raise MemoryError("oracle_csr_bp64")
+        return out
+
+
 def sd_b_proj_numpy(sino, ind, ln, off, nPixels):
     """The reference's numpy backprojector (projection/proj.py:119-150,
     ravel=True branch :133-136) over a CSR matrix: float32 volume, rays in

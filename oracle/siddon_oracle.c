@@ -333,3 +333,52 @@ int64_t oracle_fp_bp(int64_t nrays, const float *src, const float *trg,
     free(part);
     return failed ? -1 : total;
 }
+
+/*
+ * Float64 products with a stored CSR matrix (int32 column index, float32
+ * value, int64 row offsets) on `nthreads` OpenMP threads: y = A x, parallel
+ * over rows, and x = A^T y into per-thread float64 partial vectors summed
+ * in thread order (a fixed static schedule, so the result is reproducible).
+ * The textbook sparse products of projection/proj.py:56-150 in float64, for
+ * SIRT references at workload size (oracle/projection.CsrOperator).
+ * oracle_csr_bp64 returns -1 when the partial vectors cannot be allocated.
+ */
+void oracle_csr_fp64(int64_t nrows, const int32_t *ind, const float *len, const int64_t *off,
+                     const double *x, double *y, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t r = 0; r < nrows; ++r) {
+        double s = 0.0;
+        for (int64_t e = off[r]; e < off[r + 1]; ++e) s += x[ind[e]] * (double)len[e];
+        y[r] = s;
+    }
+}
+
+int oracle_csr_bp64(int64_t nrows, const int32_t *ind, const float *len, const int64_t *off,
+                    int64_t ncols, const double *y, double *x, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    double *part = (double *)calloc((size_t)nthreads * (size_t)ncols, sizeof(double));
+    if (!part) return -1;
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        const int tid = omp_get_thread_num();
+#else
+        const int tid = 0;
+#endif
+        double *mine = part + (size_t)tid * (size_t)ncols;
+#pragma omp for schedule(static)
+        for (int64_t r = 0; r < nrows; ++r) {
+            const double v = y[r];
+            for (int64_t e = off[r]; e < off[r + 1]; ++e) mine[ind[e]] += v * (double)len[e];
+        }
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t c = 0; c < ncols; ++c) {
+        double s = 0.0;
+        for (int t = 0; t < nthreads; ++t) s += part[(size_t)t * (size_t)ncols + c];
+        x[c] = s;
+    }
+    free(part);
+    return 0;
+}

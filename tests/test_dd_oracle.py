@@ -129,3 +129,15 @@ def test_cone_stale_buffer_only_matters_at_the_border():
     lhs = np.dot(fx.reshape(-1), y.reshape(-1))
     rhs = np.dot(x.reshape(-1), b.reshape(-1)) * ang.size * 1.0  # BP scale: du*dv/(na*d_pix^3)
     assert abs(lhs - rhs) <= 1e-9 * abs(lhs)
+
+
+@pytest.mark.parametrize("geom", ["par", "fan", "arc"])
+def test_sparse_overlap_matrix_equals_the_brute_force_one(geom):
+    """overlap_sparse_2d (used at workload size) is overlap_brute_2d."""
+    rng = np.random.default_rng(11)
+    ang = np.concatenate([rng.uniform(0, 2 * np.pi, 7), np.arange(8) * np.pi / 4]).astype(np.float32)
+    kw = {} if geom == "par" else dict(DSO=60.0, DSD=95.0)
+    A = dd_ref.overlap_brute_2d(13, 11, ang, 30, 1.1, 0.2, 0.9, geom=geom, **kw)
+    S = dd_ref.overlap_sparse_2d(13, 11, ang, 30, 1.1, 0.2, 0.9, geom=geom, **kw).toarray()
+    assert np.abs(A).max() > 0
+    assert np.allclose(A, S, rtol=0, atol=1e-14)

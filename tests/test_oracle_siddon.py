@@ -183,3 +183,28 @@ def test_threaded_cpu_path_matches_trace_products():
         assert nnz == ind.size
         assert np.array_equal(sino, ref_fp)
         np.testing.assert_allclose(back.reshape(-1), ref_bp.reshape(-1), rtol=1e-5, atol=1e-5)
+
+
+def test_csr_operator_equals_scipy_products():
+    """oracle.projection.CsrOperator (float64 C products used for SIRT
+    references at workload size) against scipy.sparse and fp64/bp64."""
+    import scipy.sparse as sp
+    from oracle import projection as op
+    from oracle import sirt_ref
+    rng = np.random.default_rng(21)
+    nrows, ncols = 300, 90
+    counts = rng.integers(0, 12, nrows)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    ind = rng.integers(0, ncols, off[-1]).astype(np.int32)
+    ln = rng.uniform(0, 2, off[-1]).astype(np.float32)
+    A = op.CsrOperator(ind, ln, off, ncols, nthreads=3)
+    S = sp.csr_matrix((ln.astype(np.float64), ind, off), shape=(nrows, ncols))
+    x = rng.uniform(-1, 1, ncols)
+    y = rng.uniform(-1, 1, nrows)
+    assert A.shape == (nrows, ncols) and A.T.shape == (ncols, nrows)
+    assert np.allclose(A @ x, S @ x, rtol=0, atol=1e-12)
+    assert np.allclose(A.T @ y, S.T @ y, rtol=0, atol=1e-12)
+    assert np.allclose(A @ x, op.fp64(x, ind, ln, off), rtol=0, atol=1e-12)
+    assert np.allclose(A.T @ y, op.bp64(y, ind, ln, off, ncols), rtol=0, atol=1e-12)
+    b = S @ rng.uniform(0, 1, ncols)
+    assert np.allclose(sirt_ref.sirt(A, b, 7), sirt_ref.sirt(S, b, 7), rtol=1e-12, atol=1e-14)

@@ -147,3 +147,17 @@ def test_fan_loop_matches_matrix_without_zeros_and_flags_the_stale_skip():
     stale = pd_ref.fan_fp_loop(img, ang, **kw, stale_oy=True).reshape(-1)
     np.testing.assert_allclose(fixed, pd_ref.fp(W, img, 1.1), rtol=1e-12, atol=1e-12)
     assert np.abs(stale - fixed).max() > 1e-4 * np.abs(fixed).max()
+
+
+def test_par3d_operators_equal_the_dense_matrix():
+    """par3d_ops (used at workload size) applies matrix("par3d") and its
+    transpose."""
+    rng = np.random.default_rng(12)
+    shape, nu, nv = (9, 7, 5), 20, 8
+    ang = np.linspace(0, np.pi, 6, endpoint=False)
+    W = pd_ref.matrix("par3d", shape, ang, nu, du=0.95, su=0.1, d_pix=1.3, nv=nv, dv=0.75, sv=0.25)
+    fp_, wt = pd_ref.par3d_ops(shape, ang, nu, nv, du=0.95, dv=0.75, su=0.1, sv=0.25, d_pix=1.3)
+    x = rng.uniform(0, 1, shape)
+    y = rng.uniform(0, 1, (6, nu, nv))
+    assert np.allclose(fp_(x).reshape(-1), pd_ref.fp(W, x, 1.3), rtol=0, atol=1e-13)
+    assert np.allclose(wt(y).reshape(-1), W.T @ y.reshape(-1), rtol=0, atol=1e-13)
