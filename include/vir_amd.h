@@ -139,6 +139,26 @@ int vir_siddon_bp_rec(const float *rec, int32_t nviews, int32_t nu, int32_t nv,
                       const float *sino, float *vol, uint32_t flags,
                       void *workspace, void *stream);
 
+/* vir_siddon_bp_rec without the workspace argument: the same launches and
+ * the same result, bit for bit, staging the view chunks' partial volumes
+ * (or, with VIR_ACCUMULATE, the volume) in a fixed device array of
+ * VIR_SIDDON_BP_SCRATCH_BYTES owned by the library, one per GPU for the
+ * life of the process.  The call allocates nothing (no hipMalloc/hipFree,
+ * no allocating library routine).  It returns VIR_EINVAL, launching
+ * nothing, when the call needs more staging than that: the staging is
+ * vir_siddon_bp_workspace() bytes, or nX*nY*nZ*4 with VIR_ACCUMULATE on a
+ * single-chunk plan, and every plan with more than one chunk fits.  Calls
+ * that overlap on one GPU (other threads or streams) share the array and
+ * race; serialise them.  The array keeps the last call's contents; every
+ * byte a call reads was written by that call.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, allocation-free serving loops). */
+#define VIR_SIDDON_BP_SCRATCH_BYTES (32u << 20)
+int vir_siddon_bp_rec_noalloc(const float *rec, int32_t nviews, int32_t nu, int32_t nv,
+                              const float *proj, const vir_grid *grid,
+                              const float *sino, float *vol, uint32_t flags,
+                              void *stream);
+
 /* ------------------------------------------------- stored matrix (CSR) */
 
 /* sino[r] = sum_{e in [off[r], off[r+1])} vol[ind[e]] * len[e]

@@ -163,3 +163,16 @@ def test_oracle_projection_matches_reference_c(golden):
     lhs = float(np.dot(f64, y.astype(np.float64)))
     rhs = float(np.dot(x.astype(np.float64), b64))
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_noalloc_backprojector_refuses_oversized_staging():
+    """vir_siddon_bp_rec_noalloc checks its fixed staging before anything
+    reaches the device: VIR_ACCUMULATE on a single-chunk 256^3 plan needs a
+    64 MiB volume, more than its 32 MiB (the pointers are never read)."""
+    from vir_amd import _lib
+    lib = _lib.lib()
+    g = _lib.make_grid((256, 256, 256), (1, 1, 1), (0, 0, 0), 1e-5)
+    assert lib.vir_siddon_bp_workspace(g, 4) == 0
+    p = ctypes.c_void_p(256)
+    rc = lib.vir_siddon_bp_rec_noalloc(p, 4, 8, 8, p, g, p, p, _lib.VIR_ACCUMULATE, None)
+    assert rc == -1 and b"staging" in lib.vir_last_error()

@@ -93,6 +93,7 @@ SIGNATURES = {
     "vir_siddon_ray_records": (ctypes.c_int, [P, P, I64, G, P, P]),
     "vir_siddon_fp_rec": (ctypes.c_int, [P, I64, G, P, P, U32, P, P]),
     "vir_siddon_bp_rec": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P, P]),
+    "vir_siddon_bp_rec_noalloc": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P]),
     "vir_csr_fp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
     "vir_csr_transpose": (ctypes.c_int, [P, P, P, I64, I64, I64, P, P, P, P]),
     "vir_csc_bp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdint>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "vir_common.h"
@@ -710,6 +712,66 @@ int vir_siddon_bp_rec(const float* rec, int32_t nviews, int32_t nu, int32_t nv, 
     if (!rec) return fail_arg("NULL pointer argument");
     if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
     return siddon_bp(nullptr, nullptr, rec, nviews, nu, nv, proj, grid, sino, vol, flags, workspace, stream);
+}
+
+}  // extern "C"
+
+// Staging memory of vir_siddon_bp_rec_noalloc: the view chunks' partial
+// volumes (or the volume staged for VIR_ACCUMULATE), one array per GPU for
+// the life of the process.  Host code must not use this name as a device
+// address; it uses the address hipGetSymbolAddress returns for the current
+// device (bp_scratch_ptr).
+static __device__ __align__(256) float scratch[VIR_SIDDON_BP_SCRATCH_BYTES / sizeof(float)];
+
+namespace vir {
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device ordinal, resolved once
+
+// The current device's copy of `scratch`, resolved on the first call on
+// that device (with it current).
+static int bp_scratch_ptr(void** out) {
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return VIR_OK;
+}
+
+}  // namespace vir
+
+extern "C" {
+
+int vir_siddon_bp_rec_noalloc(const float* rec, int32_t nviews, int32_t nu, int32_t nv, const float* proj,
+                              const vir_grid* grid, const float* sino, float* vol, uint32_t flags,
+                              void* stream) {
+    GridK g;
+    int rc = make_grid(grid, g);
+    if (rc) return rc;
+    if (nviews < 1) return fail_arg("nviews must be >= 1");
+    // the staging this call needs, as vir_siddon_bp_rec would take it from
+    // its workspace argument
+    const BpPlan p = plan_bp(g, nviews);
+    const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    int64_t need = 0;
+    if (p.nchunks > 1) need = (int64_t)p.nchunks * nvox * (int64_t)sizeof(float);
+    else if (flags & VIR_ACCUMULATE) need = nvox * (int64_t)sizeof(float);
+    if (need > (int64_t)VIR_SIDDON_BP_SCRATCH_BYTES)
+        return fail_arg("vir_siddon_bp_rec_noalloc: needs " + std::to_string(need) +
+                        " bytes of staging, more than its fixed " +
+                        std::to_string((int64_t)VIR_SIDDON_BP_SCRATCH_BYTES) +
+                        " (use vir_siddon_bp_rec with a workspace)");
+    void* ws = nullptr;
+    if (need > 0 && (rc = bp_scratch_ptr(&ws))) return rc;
+    return vir_siddon_bp_rec(rec, nviews, nu, nv, proj, grid, sino, vol, flags, ws, stream);
 }
 
 int vir_siddon_ray_records(const float* src, const float* trg, int64_t nrays, const vir_grid* grid, float* rec,
