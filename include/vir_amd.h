@@ -354,6 +354,18 @@ int vir_geom_circular(int32_t mode, int64_t nviews, const double *cosv,
                       int32_t nv, const double *dets_z, double src_iso,
                       double det_iso, float *src, float *trg, void *stream);
 
+/* Ray endpoints of the four-sided square geometry, generated on the device
+ * (proj_geom.py:12-57 geom_square): bin b (binsx[b], with neg_binsx[b] =
+ * -binsx[b] as the caller's dtype negates it) of side k sits at (b, -d),
+ * (-d, b), (-b, d), (d, -b) for d = dist_y and looks along angle a; the
+ * target adds the per-angle offsets step_d[a] = cos(a) d 5 and step_e[a] =
+ * sin(a) d 5 (double, as numpy evaluates them): side 0 (+D, +E), 1 (+E, +D),
+ * 2 (-D, -E), 3 (-E, -D).  src/trg: float32 (nbins, nangs, 4, 3), the
+ * float32 cast of geom_square's double endpoints. */
+int vir_geom_square(int64_t nbins, const double *binsx, const double *neg_binsx,
+                    int64_t nangs, double dist_y, const double *step_d,
+                    const double *step_e, float *src, float *trg, void *stream);
+
 /* Synthetic ellipse/ellipsoid phantom on an (nx, ny, nz) grid in
  * FOV-normalised coordinates (the style of phantoms.py): voxel value =
  * sum of rho over the ellipsoids containing its centre.  params: nell x 16

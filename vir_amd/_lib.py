@@ -115,6 +115,7 @@ SIGNATURES = {
     "vir_pdhg_primal": (ctypes.c_int, [I32, I32, I32, P, P, P, F32, P, P, P]),
     "vir_geom_circular": (ctypes.c_int, [I32, I64, P, P, I32, P, I32, P, ctypes.c_double,
                                          ctypes.c_double, P, P, P]),
+    "vir_geom_square": (ctypes.c_int, [I64, P, P, I64, ctypes.c_double, P, P, P, P, P]),
     "vir_sirt_update": (ctypes.c_int, [P, P, P, F32, I64, P]),
     "vir_phantom": (ctypes.c_int, [I32, I32, I32, I32, P, P, P]),
 }

@@ -62,6 +62,33 @@ __global__ __launch_bounds__(256) void k_geom_circular(GeomArgs A, const double*
     rot_store(A.det_iso, y, tz, c, s, trg + 3 * t);
 }
 
+// Square (four-sided translational) geometry (proj_geom.py:12-57): bin b of
+// side k at (b, -d), (-d, b), (-b, d), (d, -b), looking along angle a; the
+// target adds the per-angle offsets (D, E) = (cos a d 5, sin a d 5) given in
+// double by the host (numpy's values): side 0 (+D, +E), 1 (+E, +D),
+// 2 (-D, -E), 3 (-E, -D), in double, then rounded to float32.  Layout
+// (nbins, nangs, 4, 3).
+__global__ __launch_bounds__(256) void k_geom_square(int64_t nbins, int64_t nangs, const double* __restrict__ bx,
+                                                     const double* __restrict__ nbx, double d,
+                                                     const double* __restrict__ D,
+                                                     const double* __restrict__ E, float* __restrict__ src,
+                                                     float* __restrict__ trg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbins * nangs * 4) return;
+    const int k = (int)(t % 4);
+    const int64_t a = (t / 4) % nangs, b = t / (4 * nangs);
+    const double pb = bx[b], mb = nbx[b], dd = D[a], ee = E[a];
+    double sx, sy, tx, ty;
+    if (k == 0) { sx = pb; sy = -d; tx = sx + dd; ty = sy + ee; }
+    else if (k == 1) { sx = -d; sy = pb; tx = sx + ee; ty = sy + dd; }
+    else if (k == 2) { sx = mb; sy = d; tx = sx - dd; ty = sy - ee; }
+    else { sx = d; sy = mb; tx = sx - ee; ty = sy - dd; }
+    float* s3 = src + 3 * t;
+    float* t3 = trg + 3 * t;
+    s3[0] = (float)sx; s3[1] = (float)sy; s3[2] = 0.0f;
+    t3[0] = (float)tx; t3[1] = (float)ty; t3[2] = 0.0f;
+}
+
 }  // namespace vir
 
 using namespace vir;
@@ -83,6 +110,18 @@ int vir_geom_circular(int32_t mode, int64_t nviews, const double* cosv, const do
     hipLaunchKernelGGL(k_geom_circular, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), A,
                        cosv, sinv, dets_y, dets_z, src, trg);
     return check_hip(hipGetLastError(), "vir_geom_circular launch");
+}
+
+int vir_geom_square(int64_t nbins, const double* binsx, const double* neg_binsx, int64_t nangs,
+                    double dist_y, const double* step_d, const double* step_e, float* src, float* trg,
+                    void* stream) {
+    if (nbins < 0 || nangs < 0) return fail_arg("bad sizes");
+    const int64_t n = nbins * nangs * 4;
+    if (n == 0) return VIR_OK;
+    if (!binsx || !neg_binsx || !step_d || !step_e || !src || !trg) return fail_arg("NULL pointer argument");
+    hipLaunchKernelGGL(k_geom_square, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), nbins,
+                       nangs, binsx, neg_binsx, dist_y, step_d, step_e, src, trg);
+    return check_hip(hipGetLastError(), "vir_geom_square launch");
 }
 
 }  // extern "C"

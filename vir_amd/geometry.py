@@ -69,6 +69,27 @@ def circular_rays(DetsY, Views, geom="par", src_iso=None, det_iso=None, DetsZ=No
     return src, trg
 
 
+def square_rays(BinsX, distY, Angs, device="cuda"):
+    """Device float32 (src, trg) of shape (nBins, nAngs, 4, 3): the float32
+    cast of geom_square(BinsX, distY, Angs) (proj_geom.py:12-57), generated
+    by vir_geom_square from the bin positions and numpy's per-angle offsets
+    instead of being built on the host and copied over PCIe."""
+    from .proj_geom import square_steps
+    require_gpu()
+    BinsX = np.asarray(BinsX)
+    Angs = np.asarray(Angs)
+    D, E = square_steps(distY, Angs)
+    nb, na = BinsX.size, Angs.size
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1))).to(device)  # noqa: E731
+    bx, nbx, dd, ee = t(BinsX), t(-BinsX), t(D), t(E)
+    src = torch.empty((nb, na, 4, 3), dtype=torch.float32, device=device)
+    trg = torch.empty_like(src)
+    _lib.check(_lib.lib().vir_geom_square(nb, _lib.ptr(bx), _lib.ptr(nbx), na, float(distY), _lib.ptr(dd),
+                                          _lib.ptr(ee), _lib.ptr(src), _lib.ptr(trg), stream()),
+               "vir_geom_square")
+    return src, trg
+
+
 def _uniform(a):
     if a.size < 3:
         return True

@@ -9,23 +9,50 @@ a voxel brick's detector footprint.
 import numpy as np
 
 
+def square_steps(distY, Angs):
+    """Per-angle target offsets (D, E) = (cos(a) distY 5, sin(a) distY 5) of
+    geom_square, evaluated with numpy in the input dtypes as the reference
+    does (proj_geom.py:51-54): float32 angles give float32 offsets."""
+    return np.cos(Angs) * distY * 5, np.sin(Angs) * distY * 5
+
+
 def geom_square(BinsX, distY, Angs):
     """Source/target coordinates of a four-sided translational fan geometry
-    (proj_geom.py:12-57).  Returns float64 arrays (nBins, nAngs, 4, 3)."""
-    nBins = BinsX.size
-    nAngs = Angs.size
-    srcs = np.zeros([nBins, nAngs, 4, 3])
-    srcs[..., [0, 1], [0, 1]] = np.repeat(BinsX, nAngs * 2).reshape((nBins, nAngs, 2))
-    srcs[..., [2, 3], [0, 1]] = np.repeat(-BinsX, nAngs * 2).reshape((nBins, nAngs, 2))
-    srcs[..., [0, 1], [1, 0]] = np.broadcast_to(-distY, (nBins, nAngs, 2))
-    srcs[..., [2, 3], [1, 0]] = np.broadcast_to(distY, (nBins, nAngs, 2))
-    srcs[..., 2] = np.broadcast_to(0.0, (nBins, nAngs, 4))
-    trgs = srcs.copy()
-    trgs[..., [0, 1], [0, 1]] += (np.cos(Angs) * distY * 5)[..., np.newaxis]
-    trgs[..., [2, 3], [0, 1]] -= (np.cos(Angs) * distY * 5)[..., np.newaxis]
-    trgs[..., [0, 1], [1, 0]] += (np.sin(Angs) * distY * 5)[..., np.newaxis]
-    trgs[..., [2, 3], [1, 0]] -= (np.sin(Angs) * distY * 5)[..., np.newaxis]
-    return srcs, trgs
+    (proj_geom.py:12-57), float64 arrays (nBins, nAngs, 4, 3), z = 0.
+
+    Side k of the square holds the detectors at BinsX b along one edge, at
+    distance distY from the centre; each looks across the square at angle a:
+
+        side   source        target
+        0      ( b, -d)      ( b + D, -d + E)
+        1      (-d,  b)      (-d + E,  b + D)
+        2      (-b,  d)      (-b - D,  d - E)
+        3      ( d, -b)      ( d - E, -b - D)
+
+    with d = distY and (D, E) = square_steps(distY, a); every coordinate is
+    the source value plus or minus one offset in float64, so the endpoints
+    are the reference's bits (tests/golden/geom_square_golden.npz)."""
+    BinsX = np.asarray(BinsX)
+    Angs = np.asarray(Angs)
+    nBins, nAngs = BinsX.size, Angs.size
+    D, E = square_steps(distY, Angs)
+    D = np.asarray(D, np.float64)[None, :]
+    E = np.asarray(E, np.float64)[None, :]
+    b = BinsX.reshape(-1).astype(np.float64)[:, None]
+    nb = (-BinsX).reshape(-1).astype(np.float64)[:, None]
+    d = np.float64(distY)
+    md = np.float64(-distY)
+    src = np.zeros((nBins, nAngs, 4, 3))
+    trg = np.zeros((nBins, nAngs, 4, 3))
+    sides = ((b, md, +1, +1), (md, b, +1, +1), (nb, d, -1, -1), (d, nb, -1, -1))
+    for k, (sx, sy, sgn_d, sgn_e) in enumerate(sides):
+        src[:, :, k, 0] = sx
+        src[:, :, k, 1] = sy
+        # side 0/2 move x by D and y by E; side 1/3 move x by E and y by D
+        dx, dy = (D, E) if k % 2 == 0 else (E, D)
+        trg[:, :, k, 0] = sx + dx if sgn_d > 0 else sx - dx
+        trg[:, :, k, 1] = sy + dy if sgn_e > 0 else sy - dy
+    return src, trg
 
 
 def geom_circular(DetsY, Views, geom="par", src_iso=None, det_iso=None, DetsZ=None):
