@@ -402,6 +402,22 @@ int vir_tv(const float *x, int64_t nbatch, int32_t nx, int32_t ny, int32_t nz,
            float eps, float weight, float *grad, double *value, double *work,
            uint32_t flags, void *stream);
 
+/* Total-variation denoising by Chambolle's projection algorithm (J. Math.
+ * Imaging Vis. 20 (2004) 89-97, eq. (9)): u = argmin ||u - g||^2/2 + weight
+ * TV(u) (isotropic TV, forward differences zero at the last index) on an
+ * image of ndim = 1..3 axes, sizes nx, ny, nz (1 for missing axes), last
+ * axis fastest.  Runs `iters` iterations of the dual update on q (ndim
+ * planes of the image's size, in/out; start from zeros) with step tau
+ * (<= 1/(4 ndim) converges), using v (one image) as scratch; then, when u
+ * is not NULL, writes u = g - div q.  dmax (may be NULL): VIR_TV_WORK
+ * floats of work; dmax[0] receives the largest |q change| of the last
+ * iteration, for a stopping rule.  The denoiser the reference applies to
+ * SIRT iterates (tests/Test_SIRT/sirt_test.py:13,38, scikit-image's
+ * denoise_tv_chambolle). */
+int vir_tv_chambolle(int32_t ndim, int32_t nx, int32_t ny, int32_t nz,
+                     const float *g, float *q, float *v, float *u, float weight,
+                     float tau, int32_t iters, float *dmax, void *stream);
+
 /* Chambolle-Pock iteration for TV-regularised reconstruction (tv.py:229-294),
  * on volumes (nx, ny, nz) with periodic differences (tv.py:207-224); nz == 1
  * is 2-D.  z holds ndim (2 or 3) planes of the volume's size.

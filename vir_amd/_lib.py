@@ -110,6 +110,7 @@ SIGNATURES = {
     "vir_sirt_residual": (ctypes.c_int, [P, P, P, I64, P]),
     "vir_tv": (ctypes.c_int, [P, I64, I32, I32, I32, F32, F32, P, P, P, U32, P]),
     "vir_sqdist": (ctypes.c_int, [P, P, P, I64, P, P, P]),
+    "vir_tv_chambolle": (ctypes.c_int, [I32, I32, I32, I32, P, P, P, P, F32, F32, I32, P, P]),
     "vir_pdhg_dual_tv": (ctypes.c_int, [I32, I32, I32, P, P, F32, P]),
     "vir_pdhg_dual_data": (ctypes.c_int, [I64, P, P, P, F32, P, P]),
     "vir_pdhg_primal": (ctypes.c_int, [I32, I32, I32, P, P, P, F32, P, P, P]),

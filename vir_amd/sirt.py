@@ -146,11 +146,12 @@ class SIRT:
         return self.x
 
 
-def sirt(sino, sdlist, W_sino, W_img, nPixels, gamma=1, beta=0, iters=10):
+def sirt(sino, sdlist, W_sino, W_img, nPixels, gamma=1, beta=0, iters=10, tv_kw=None):
     """Drop-in for tests/Test_SIRT/sirt_test.py:21-40: x0 = 0, `iters`
     updates, returns every iterate as float32 array nPixels + (iters,).
-    beta > 0 applies total-variation denoising after each update
-    (sirt_test.py:37-38)."""
+    beta > 0 applies total-variation denoising with weight beta after each
+    update (sirt_test.py:37-38), vir_amd.tv.denoise_tv_chambolle with its
+    defaults or the keyword arguments in tv_kw."""
     nPixels = tuple(int(v) for v in np.atleast_1d(nPixels))
     P = StoredMatrix(sdlist, nPixels, ray_shape=np.shape(sino))
     S = SIRT(P, sino, W_sino=W_sino, W_img=W_img, gamma=gamma)
@@ -159,7 +160,7 @@ def sirt(sino, sdlist, W_sino, W_img, nPixels, gamma=1, beta=0, iters=10):
         S.step()
         if beta > 0:
             from .tv import denoise_tv_chambolle
-            S.x.copy_(denoise_tv_chambolle(S.x, weight=beta))
+            S.x.copy_(denoise_tv_chambolle(S.x, weight=beta, **(tv_kw or {})))
         out[..., i] = S.x.cpu().numpy()
     return out
 
