@@ -36,6 +36,9 @@ extern "C" {
                                for these rays and grid */
 #define VIR_PER_OUTPUT 8u   /* DD 2-D / SF: the one-thread-per-output kernels
                                instead of the tiled ones (same bits) */
+#define VIR_SF_UNIT_AMPLITUDE 16u  /* SF parallel modes: footprint amplitude 1
+                               instead of 1/(|sin|+|cos|) -- the rectangle
+                               footprint of fp_2d_par.py:67-125 */
 
 /* Voxel grid: Grid3d(nPixels, dPixels, origin, float32) of
  * core/core_funcs.py:235-265 plus the snapping epsilon of
@@ -284,6 +287,52 @@ int vir_sf_fp(const vir_sf_geom *geom, const float *ang, const float *cosv,
 int vir_sf_bp(const vir_sf_geom *geom, const float *ang, const float *cosv,
               const float *sinv, const float *sino, float *img, float out_scale,
               uint32_t flags, void *stream);
+
+/* -------------------------------- separable-footprint prototypes (sp.py) */
+
+/* Triangle (linear B-spline) footprint fan beam of sys_mat/sp.py:12-87:
+ * image (nx, ny) of pixels d_pix centred at the origin, na views with
+ * per-view cos/sin (device), nu bins of pitch du with centres
+ * du (b - nu/2 + 1/2).  A pixel centre (x, y) projects to u0 = DSD xp /
+ * (DSO - yp) with xp = c x + s y, yp = -s x + c y, and weighs bin b by
+ * max(0, 1 - |u_b - u0| / h), h = DSD d_pix / (DSO - yp). */
+typedef struct vir_sp_geom {
+    int32_t nx, ny, nu, na;
+    float d_pix, du, DSO, DSD;
+} vir_sp_geom;
+
+/* sino (na, nu) = out_scale * sum over pixels of img * weight; the
+ * reference's sf_fp_fan_2d uses out_scale = d_pix (sp.py:87).  One thread
+ * per bin. */
+int vir_sp_fan_fp(const vir_sp_geom *geom, const float *cosv, const float *sinv,
+                  const float *img, float *sino, float out_scale, uint32_t flags,
+                  void *stream);
+
+/* Exact transpose of vir_sp_fan_fp (same weights), one thread per pixel.
+ * No reference counterpart (the reference prototype has no backprojector);
+ * the adjoint of sf_fp_fan_2d. */
+int vir_sp_fan_bp(const vir_sp_geom *geom, const float *cosv, const float *sinv,
+                  const float *sino, float *img, float out_scale, uint32_t flags,
+                  void *stream);
+
+/* The separable toy model of sys_mat/sp.py:93-211 (CTGeometry,
+ * footprint1D, forward_project, back_project): voxel centres (i - n/2 +
+ * 1/2) d along each axis, detector centres (j - n/2 + 1/2) pitch, Ds the
+ * source-isocentre and Dd the source-detector distance, f(r) = max(0,
+ * 1 - |r|):
+ *   proj[v, u] = sum_{x,y,z} vol[x,y,z] f(u/Dd - x/Ds) f(v/Dd - y/Ds) f(z/Ds)
+ * (forward_project) and its transpose (back_project), in double.  work:
+ * vir_sep_workspace() bytes of device scratch. */
+typedef struct vir_sep_geom {
+    int32_t nx, ny, nz, nu, nv;
+    double dx, dy, dz, su, sv, Ds, Dd;
+} vir_sep_geom;
+
+int64_t vir_sep_workspace(const vir_sep_geom *geom);
+int vir_sep_fp(const vir_sep_geom *geom, const float *vol, float *proj, void *work,
+               void *stream);
+int vir_sep_bp(const vir_sep_geom *geom, const float *proj, float *vol, void *work,
+               void *stream);
 
 /* ------------------------------------------ filtered backprojection */
 

@@ -16,6 +16,7 @@ VIR_ACCUMULATE = 1
 VIR_SEQUENTIAL = 2
 VIR_UNGUARDED = 4
 VIR_PER_OUTPUT = 8
+VIR_SF_UNIT_AMPLITUDE = 16
 SIDDON_RECORD_BYTES = 80  # include/vir_amd.h: VIR_SIDDON_RECORD_BYTES
 
 P = ctypes.c_void_p
@@ -55,6 +56,21 @@ class VirSFGeom(ctypes.Structure):
 
 
 SG = ctypes.POINTER(VirSFGeom)
+
+
+class VirSpGeom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nu", ctypes.c_int32),
+                ("na", ctypes.c_int32), ("d_pix", ctypes.c_float), ("du", ctypes.c_float),
+                ("DSO", ctypes.c_float), ("DSD", ctypes.c_float)]
+
+
+class VirSepGeom(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("nx", "ny", "nz", "nu", "nv")] + \
+               [(k, ctypes.c_double) for k in ("dx", "dy", "dz", "su", "sv", "Ds", "Dd")]
+
+
+SPG = ctypes.POINTER(VirSpGeom)
+SEPG = ctypes.POINTER(VirSepGeom)
 
 
 class VirRDGeom(ctypes.Structure):
@@ -107,6 +123,11 @@ SIGNATURES = {
     "vir_rd_bp": (ctypes.c_int, [RG, P, P, P, P, F32, U32, P]),
     "vir_sf_fp": (ctypes.c_int, [SG, P, P, P, P, P, F32, U32, P]),
     "vir_sf_bp": (ctypes.c_int, [SG, P, P, P, P, P, F32, U32, P]),
+    "vir_sp_fan_fp": (ctypes.c_int, [SPG, P, P, P, P, F32, U32, P]),
+    "vir_sp_fan_bp": (ctypes.c_int, [SPG, P, P, P, P, F32, U32, P]),
+    "vir_sep_workspace": (I64, [SEPG]),
+    "vir_sep_fp": (ctypes.c_int, [SEPG, P, P, P, P]),
+    "vir_sep_bp": (ctypes.c_int, [SEPG, P, P, P, P]),
     "vir_sirt_residual": (ctypes.c_int, [P, P, P, I64, P]),
     "vir_tv": (ctypes.c_int, [P, I64, I32, I32, I32, F32, F32, P, P, P, U32, P]),
     "vir_sqdist": (ctypes.c_int, [P, P, P, I64, P, P, P]),

@@ -30,6 +30,7 @@ struct SFGeo {
     float d, du, dv, su, sv, DSO, DSD;
     float out_scale;
     float wmax;  // largest footprint half-width along u (detector units)
+    int unit_amp;  // parallel modes: amplitude 1 (VIR_SF_UNIT_AMPLITUDE)
 };
 
 __device__ __forceinline__ float sf_bnd(int n, float d, int k) { return d * ((float)k - 0.5f * (float)n); }
@@ -56,7 +57,7 @@ __device__ __forceinline__ SFView sf_view(const SFGeo& G, float c, float s, floa
     V.s = s;
     V.ang = ang;
     const float w = fabsf(s) + fabsf(c);
-    V.amp = 1.0f / w;
+    V.amp = G.unit_amp ? 1.0f : 1.0f / w;
     V.h = 0.5f * G.d * w;
     return V;
 }
@@ -737,6 +738,8 @@ static int sf_setup(const vir_sf_geom* g, SFGeo& G) {
     G.nu = g->nu; G.nv = is3 ? g->nv : 1; G.na = g->na; G.nb = g->nbatch;
     G.d = g->d_pix; G.du = g->du; G.dv = is3 ? g->dv : 1.0f;
     G.su = g->su; G.sv = is3 ? g->sv : 0.0f; G.DSO = g->DSO; G.DSD = g->DSD;
+    G.out_scale = 1.0f;
+    G.unit_amp = 0;
     if (G.nx < 1 || G.ny < 1 || G.nz < 1 || G.nu < 1 || G.nv < 1 || G.na < 0 || G.nb < 1)
         return fail_arg("bad sizes");
     if (!(G.d > 0.0f) || !(G.du > 0.0f) || !(G.dv > 0.0f)) return fail_arg("spacings must be > 0");
@@ -784,6 +787,7 @@ int vir_sf_fp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
     if (G.na == 0) return VIR_OK;
     if (!ang || !cosv || !sinv || !img || !sino) return fail_arg("NULL pointer argument");
     G.out_scale = out_scale;
+    G.unit_amp = (flags & VIR_SF_UNIT_AMPLITUDE) ? 1 : 0;
     int64_t n = (int64_t)G.nb * G.na * G.nu * G.nv;
     dim3 grd((unsigned)((n + 255) / 256));
     hipStream_t st = as_stream(stream);
@@ -813,6 +817,7 @@ int vir_sf_bp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
     if (rc) return rc;
     if ((G.na > 0 && (!ang || !cosv || !sinv || !sino)) || !img) return fail_arg("NULL pointer argument");
     G.out_scale = out_scale;
+    G.unit_amp = (flags & VIR_SF_UNIT_AMPLITUDE) ? 1 : 0;
     int64_t n = (int64_t)G.nb * G.nx * G.ny * G.nz;
     dim3 grd((unsigned)((n + 255) / 256));
     hipStream_t st = as_stream(stream);

@@ -45,8 +45,13 @@ class SFProjector:
     > 1).  Angles in radians; cos/sin are rounded to float32."""
 
     def __init__(self, img_shape, ang_arr, nu, geom="par", nv=1, DSO=0.0, DSD=0.0, du=1.0,
-                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, batch=1):
+                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, batch=1, unit_amplitude=False):
+        """unit_amplitude (parallel geometries): footprint amplitude 1 instead
+        of 1/(|sin| + |cos|), the rectangle footprint of fp_2d_par.py:67-125."""
         require_gpu()
+        if unit_amplitude and geom not in ("par", "par3d"):
+            raise ValueError("unit_amplitude applies to the parallel geometries")
+        self._flags = _lib.VIR_SF_UNIT_AMPLITUDE if unit_amplitude else 0
         if geom not in _MODES:
             raise ValueError(f"unknown geometry {geom!r}")
         ang64 = np.asarray(ang_arr, dtype=np.float64).reshape(-1)
@@ -78,7 +83,7 @@ class SFProjector:
         self.ref_bp_scale = float(s * self.dv if geom == "cone" else s)
 
     def _call(self, name, src, dst, scale, accumulate):
-        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._flags
         _lib.check(getattr(_lib.lib(), name)(self.g, _lib.ptr(self.ang), _lib.ptr(self.cos),
                                              _lib.ptr(self.sin), _lib.ptr(src), _lib.ptr(dst),
                                              scale, fl, stream()), name)
