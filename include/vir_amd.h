@@ -162,6 +162,48 @@ int vir_siddon_bp_rec_noalloc(const float *rec, int32_t nviews, int32_t nu, int3
                               const float *sino, float *vol, uint32_t flags,
                               void *stream);
 
+/* ---------------------------------------------- siddons.c (single ray) */
+
+/* Grid of sys_mat/siddons.c: nPixels (VectI) and dPixels (VectF), with the
+ * grid lines x_lines (nx+1), y_lines (ny+1), z_lines (nz+1) as its
+ * calc_grid_lines writes them (siddons.c:137-148: -n d / 2, then repeated
+ * float additions of d), given as device arrays. */
+typedef struct vir_sc_grid {
+    int32_t nx, ny, nz;
+    float dx, dy, dz;
+} vir_sc_grid;
+
+/* siddons() of sys_mat/siddons.c:290-341 for nrays rays at once (src, trg:
+ * (nrays, 3) float32): counts[r] = the number of VoxelLength rows of ray r
+ * (0 when it misses the grid; siddons.c then leaves its row untouched).
+ * Same float32 steps as siddons.c, including its range that ignores the z
+ * slab (max4/min4, :109-134) and its x-fastest index nx ny z + nx y + x
+ * (:253); planes outside [0, N], where siddons.c reads outside its arrays,
+ * are not merged. */
+int vir_siddons_c_count(const float *src, const float *trg, int64_t nrays,
+                        const vir_sc_grid *grid, const float *x_lines,
+                        const float *y_lines, const float *z_lines,
+                        int32_t *counts, void *stream);
+
+/* The rows themselves: ray r's (idx, length) at [offsets[r], offsets[r+1])
+ * (siddons.c:235-261, without the idx = -1 terminator). */
+int vir_siddons_c_fill(const float *src, const float *trg, int64_t nrays,
+                       const vir_sc_grid *grid, const float *x_lines,
+                       const float *y_lines, const float *z_lines,
+                       const int64_t *offsets, int32_t *idx, float *len,
+                       void *stream);
+
+/* siddons_proj() (siddons.c:345-396, sum_inter :265-287) for nrays rays:
+ * ray_sum[r] = sum of vol[idx] * |dST| * dalpha in float32, entry order.
+ * vol is (nz, ny, nx) with x fastest.  Entries outside the grid (undefined
+ * reads in siddons.c) add nothing; err_count (may be NULL) receives their
+ * number. */
+int vir_siddons_c_proj(const float *src, const float *trg, int64_t nrays,
+                       const vir_sc_grid *grid, const float *x_lines,
+                       const float *y_lines, const float *z_lines,
+                       const float *vol, float *ray_sum, int32_t *err_count,
+                       void *stream);
+
 /* ------------------------------------------------- stored matrix (CSR) */
 
 /* sino[r] = sum_{e in [off[r], off[r+1])} vol[ind[e]] * len[e]

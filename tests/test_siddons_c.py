@@ -1,0 +1,88 @@
+"""The single-ray C Siddon (sys_mat/siddons.c) against the reference's own
+siddons.c compiled unmodified (oracle/siddons_c_ref.py): grid lines on the
+host (CPU), then on the GPU every ray's rows (x-fastest index, length) and
+ray sums bit for bit, and the file's 64-ray forward_project demonstration.
+
+Rays are kept where siddons.c stays inside its arrays: they cross the z
+slab's interior between their x-y entry and exit (its range ignores z), and
+projected rays do not run along an x or y plane outside the grid."""
+import numpy as np
+import pytest
+
+from oracle import siddons_c_ref
+from vir_amd.sys_mat import siddons_c
+
+
+def _ref(nP, dP):
+    try:
+        return siddons_c_ref.SiddonsC(nP, dP)
+    except FileNotFoundError:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+
+
+@pytest.mark.parametrize("nP,dP", [((13, 11, 9), (1.0, 0.7, 1.3)), ((6, 6, 8), 1.0), ((40, 3, 17), (0.3, 1.1, 0.9))])
+def test_grid_lines_are_the_reference(nP, dP):
+    R = _ref(nP, dP)
+    for mine, ref in zip(siddons_c.calc_grid_lines(nP, dP), R.lines):
+        assert np.array_equal(mine.view(np.int32), ref.view(np.int32))
+
+
+def _rays(rng, nP, dP, n):
+    d = np.broadcast_to(np.asarray(dP, np.float64), (3,))
+    half = np.asarray(nP) * d / 2
+    src, trg = [], []
+    for k in range(n):
+        th = rng.uniform(0, 2 * np.pi)
+        c, s = np.cos(th), np.sin(th)
+        r = 1.6 * max(half[0], half[1])
+        off = rng.uniform(-0.9, 0.9) * min(half[0], half[1])
+        p0 = np.array([-r * c - off * s, -r * s + off * c])
+        p1 = np.array([r * c - off * s, r * s + off * c])
+        z0, z1 = rng.uniform(-0.8, 0.8, 2) * half[2]
+        src.append([p0[0], p0[1], z0])
+        trg.append([p1[0], p1[1], z1])
+    src, trg = np.array(src, np.float32), np.array(trg, np.float32)
+    # flat (dz = 0), rays that miss the square, and rays along an axis inside it
+    src[:8, 2] = trg[:8, 2]
+    src[8:12, :2] += 5 * half[:2].max()
+    trg[8:12, :2] += 5 * half[:2].max()
+    src[12:16, 0] = trg[12:16, 0] = rng.uniform(-0.9, 0.9, 4) * half[0]
+    return src, trg
+
+
+CASES = [((13, 11, 9), (1.0, 0.7, 1.3)), ((32, 32, 4), 1.0), ((20, 16, 24), (0.5, 0.75, 0.4))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nP,dP", CASES)
+def test_rows_and_sums_are_the_reference(nP, dP):
+    R = _ref(nP, dP)
+    rng = np.random.default_rng(sum(nP))
+    src, trg = _rays(rng, nP, dP, 300)
+    idx, ln, off = siddons_c.siddons(src, trg, nP, dP)
+    vol = rng.uniform(0, 1, (nP[2], nP[1], nP[0])).astype(np.float32)   # x fastest
+    sums = siddons_c.siddons_proj(vol, src, trg, nP, dP)
+    nonempty = 0
+    for r in range(src.shape[0]):
+        ri, rl = R.ray(src[r], trg[r])
+        gi, gl = idx[off[r]:off[r + 1]], ln[off[r]:off[r + 1]]
+        assert np.array_equal(gi, ri), r
+        assert np.array_equal(gl.view(np.int32), rl.view(np.int32)), r
+        assert np.float32(sums[r]).view(np.int32) == R.proj(vol, src[r], trg[r]).view(np.int32), r
+        nonempty += ri.size > 0
+    assert nonempty >= 250
+
+
+@pytest.mark.gpu
+def test_single_ray_call_and_forward_project_demo():
+    nP, dP = (6, 6, 8), 1.0
+    R = _ref(nP, dP)
+    gi, gl = siddons_c.siddon_c([-9.0, 0.3, 0.2], [9.0, -0.5, 0.6], nP, dP)
+    ri, rl = R.ray([-9.0, 0.3, 0.2], [9.0, -0.5, 0.6])
+    assert ri.size > 0 and np.array_equal(gi, ri) and np.array_equal(gl, rl)
+    vol = np.random.default_rng(1).uniform(0, 1, 6 * 6 * 8).astype(np.float32)
+    out = np.zeros(64, np.float32)
+    siddons_c.forward_project_c(vol, out, nP, dP)
+    ref = R.forward_project(vol)
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(out.view(np.int32), ref.view(np.int32))

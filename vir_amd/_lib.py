@@ -69,6 +69,12 @@ class VirSepGeom(ctypes.Structure):
                [(k, ctypes.c_double) for k in ("dx", "dy", "dz", "su", "sv", "Ds", "Dd")]
 
 
+class VirScGrid(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("nx", "ny", "nz")] + \
+               [(k, ctypes.c_float) for k in ("dx", "dy", "dz")]
+
+
+SCG = ctypes.POINTER(VirScGrid)
 SPG = ctypes.POINTER(VirSpGeom)
 SEPG = ctypes.POINTER(VirSepGeom)
 
@@ -110,6 +116,9 @@ SIGNATURES = {
     "vir_siddon_fp_rec": (ctypes.c_int, [P, I64, G, P, P, U32, P, P]),
     "vir_siddon_bp_rec": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P, P]),
     "vir_siddon_bp_rec_noalloc": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P]),
+    "vir_siddons_c_count": (ctypes.c_int, [P, P, I64, SCG, P, P, P, P, P]),
+    "vir_siddons_c_fill": (ctypes.c_int, [P, P, I64, SCG, P, P, P, P, P, P, P]),
+    "vir_siddons_c_proj": (ctypes.c_int, [P, P, I64, SCG, P, P, P, P, P, P, P]),
     "vir_csr_fp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
     "vir_csr_transpose": (ctypes.c_int, [P, P, P, I64, I64, I64, P, P, P, P]),
     "vir_csc_bp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvir_amd.so")
 OBJ = os.path.join(HERE, "_build")
 SOURCES = ["vir_common.hip", "siddon.hip", "csr.hip", "sirt_ops.hip", "dd.hip", "sf.hip", "loss.hip", "geom.hip",
-           "phantom.hip", "rd.hip", "sq.hip", "pdhg.hip", "fbp.hip", "tvden.hip", "sp.hip"]
+           "phantom.hip", "rd.hip", "sq.hip", "pdhg.hip", "fbp.hip", "tvden.hip", "sp.hip", "siddons_c.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contract=off",
