@@ -4,12 +4,16 @@ only.  ctypes calls of its calc_grid_lines (:137-148), siddons (:290-341),
 siddons_proj (:345-396) and forward_project (:403-460), one ray per call as
 the C API is.  Its debug printf output is sent to /dev/null.
 
-The scratch arrays are sized with room to spare (the reference's own
-callers size x_alpha at N+1 values although calc_alphas writes up to N+2),
-so calls stay inside their buffers; callers must keep the rays where
-siddons.c itself stays inside its plane arrays (rays whose plane index
-range lies in [0, N] on every axis) and, for forward_project, nz >= nx/2 +
-ny/2 + 1 (its merged list buffer holds 2 nz + 3 values).
+siddons.c reads outside its grid-line arrays on some rays: its first plane
+index floor(N - (plane[N] - entry)/d) is -1 when rounding puts a ray's entry
+a hair before plane 0, and its range ignores the z slab.  The grid lines
+are therefore held between guard values, every call runs twice with two
+different guards, and a result that changes with the guard is reported as
+not in bounds (undefined in siddons.c).  Scratch arrays are sized with room
+to spare (the reference's own callers size x_alpha at N+1 values although
+calc_alphas writes up to N+2).  forward_project allocates its own arrays:
+callers use it only on rays the per-ray calls report in bounds, and with
+nz >= nx/2 + ny/2 + 1 (its merged list buffer holds 2 nz + 3 values).
 """
 import contextlib
 import ctypes
@@ -62,7 +66,9 @@ class SiddonsC:
         self.n = VectI(*(int(v) for v in n))
         self.d = VectF(*(float(v) for v in d))
         self.shape = tuple(int(v) for v in n)
-        self.lines = [np.zeros(int(v) + 1, np.float32) for v in n]
+        # grid lines at [1, N+2) of buffers with one guard value on each side
+        self._guarded = [np.zeros(int(v) + 3, np.float32) for v in n]
+        self.lines = [b[1:-1] for b in self._guarded]
         self.so.calc_grid_lines.restype = None
         self.so.calc_grid_lines(*(a.ctypes.data_as(FP) for a in self.lines), self.n, self.d)
         m = int(n.sum()) + 8
@@ -77,9 +83,36 @@ class SiddonsC:
         return ([a.ctypes.data_as(FP) for a in self.lines] + [a.ctypes.data_as(FP) for a in self.alpha] +
                 [self.all_alpha.ctypes.data_as(FP)])
 
+    def _guard(self, value):
+        for b in self._guarded:
+            b[0] = b[-1] = value
+
+    def _twice(self, call):
+        """call() with both guards; (result, in_bounds)."""
+        self._guard(np.float32(-3.0e4))
+        a = call()
+        self._guard(np.float32(7.5e3))
+        b = call()
+        same = all(np.array_equal(np.asarray(x).view(np.int32), np.asarray(y).view(np.int32))
+                   for x, y in zip(a if isinstance(a, tuple) else (a,), b if isinstance(b, tuple) else (b,)))
+        return a, same
+
     def ray(self, src, trg):
-        """The rows of one ray: (idx int32, length float32); empty when
-        siddons() leaves the row buffer untouched (a ray that misses)."""
+        """((idx int32, length float32), in_bounds) of one ray; rows empty
+        when siddons() leaves the row buffer untouched (a ray that misses)."""
+        return self._twice(lambda: self._ray(src, trg))
+
+    def proj(self, iArray, src, trg):
+        """(ray sum float32, in_bounds) of one ray; siddons_proj is called
+        only when the ray's rows are in bounds and every index lies inside
+        the volume (its reads are then defined), else (None, False)."""
+        (idx, _), ok = self.ray(src, trg)
+        nvox = self.shape[0] * self.shape[1] * self.shape[2]
+        if not ok or (idx.size and (idx.min() < 0 or idx.max() >= nvox)):
+            return None, False
+        return self._proj(iArray, src, trg), True
+
+    def _ray(self, src, trg):
         for k in range(len(self.rows)):
             self.rows[k].idx, self.rows[k].length = -7, float("nan")
         with _quiet():
@@ -94,7 +127,7 @@ class SiddonsC:
         ln = np.array([self.rows[i].length for i in range(k)], np.float32)
         return idx, ln
 
-    def proj(self, iArray, src, trg):
+    def _proj(self, iArray, src, trg):
         vol = np.ascontiguousarray(np.asarray(iArray, np.float32).reshape(-1))
         out = ctypes.c_float(0.0)
         with _quiet():

@@ -3,9 +3,12 @@ siddons.c compiled unmodified (oracle/siddons_c_ref.py): grid lines on the
 host (CPU), then on the GPU every ray's rows (x-fastest index, length) and
 ray sums bit for bit, and the file's 64-ray forward_project demonstration.
 
-Rays are kept where siddons.c stays inside its arrays: they cross the z
-slab's interior between their x-y entry and exit (its range ignores z), and
-projected rays do not run along an x or y plane outside the grid."""
+siddons.c reads outside its arrays on many rays (its first plane index is
+-1 when a ray's entry rounds a hair before plane 0; its range ignores the z
+slab), where its result is undefined.  The oracle reports those rays (the
+result changes with guard values around the grid lines) and they are left
+out; the others -- about half of the rays, required below -- must match
+exactly."""
 import numpy as np
 import pytest
 
@@ -62,15 +65,20 @@ def test_rows_and_sums_are_the_reference(nP, dP):
     idx, ln, off = siddons_c.siddons(src, trg, nP, dP)
     vol = rng.uniform(0, 1, (nP[2], nP[1], nP[0])).astype(np.float32)   # x fastest
     sums = siddons_c.siddons_proj(vol, src, trg, nP, dP)
-    nonempty = 0
+    checked = summed = 0
     for r in range(src.shape[0]):
-        ri, rl = R.ray(src[r], trg[r])
+        (ri, rl), ok = R.ray(src[r], trg[r])
+        if not ok:
+            continue
         gi, gl = idx[off[r]:off[r + 1]], ln[off[r]:off[r + 1]]
         assert np.array_equal(gi, ri), r
         assert np.array_equal(gl.view(np.int32), rl.view(np.int32)), r
-        assert np.float32(sums[r]).view(np.int32) == R.proj(vol, src[r], trg[r]).view(np.int32), r
-        nonempty += ri.size > 0
-    assert nonempty >= 250
+        checked += ri.size > 0
+        ref_sum, ok = R.proj(vol, src[r], trg[r])
+        if ok:
+            assert np.float32(sums[r]).view(np.int32) == ref_sum.view(np.int32), r
+            summed += 1
+    assert checked >= 100 and summed >= 100, (checked, summed)
 
 
 @pytest.mark.gpu
@@ -78,11 +86,15 @@ def test_single_ray_call_and_forward_project_demo():
     nP, dP = (6, 6, 8), 1.0
     R = _ref(nP, dP)
     gi, gl = siddons_c.siddon_c([-9.0, 0.3, 0.2], [9.0, -0.5, 0.6], nP, dP)
-    ri, rl = R.ray([-9.0, 0.3, 0.2], [9.0, -0.5, 0.6])
-    assert ri.size > 0 and np.array_equal(gi, ri) and np.array_equal(gl, rl)
+    (ri, rl), ok = R.ray([-9.0, 0.3, 0.2], [9.0, -0.5, 0.6])
+    assert ok and ri.size > 0 and np.array_equal(gi, ri) and np.array_equal(gl, rl)
     vol = np.random.default_rng(1).uniform(0, 1, 6 * 6 * 8).astype(np.float32)
     out = np.zeros(64, np.float32)
     siddons_c.forward_project_c(vol, out, nP, dP)
     ref = R.forward_project(vol)
     assert np.abs(ref).max() > 0
-    assert np.array_equal(out.view(np.int32), ref.view(np.int32))
+    # the demonstration's rays whose reads are defined in siddons.c
+    ds, dt = siddons_c.demo_rays(nP, dP)
+    ok = np.array([R.proj(vol, ds[k], dt[k])[1] for k in range(64)])
+    assert ok.sum() >= 40
+    assert np.array_equal(out[ok].view(np.int32), ref[ok].view(np.int32))

