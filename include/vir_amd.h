@@ -39,6 +39,13 @@ extern "C" {
 #define VIR_SF_UNIT_AMPLITUDE 16u  /* SF parallel modes: footprint amplitude 1
                                instead of 1/(|sin|+|cos|) -- the rectangle
                                footprint of fp_2d_par.py:67-125 */
+#define VIR_COMPAT 32u      /* reproduce a reference quirk instead of the
+                               intended model: ray-driven FP/BP clamp rays
+                               parallel to an axis outside the image onto the
+                               border row (rd.py:66-73); SF fan FP keeps a
+                               stale o boundary after zero pixels
+                               (pd.py:321-325); cone DD FP re-zeroes only part
+                               of its u buffer (dd.py:317, 374) */
 
 /* Voxel grid: Grid3d(nPixels, dPixels, origin, float32) of
  * core/core_funcs.py:235-265 plus the snapping epsilon of
@@ -260,12 +267,15 @@ int vir_dd2d_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
                 void *stream);
 
 /* Cone-beam DD (dd.py:284-468, 815-888): sino (na, nu, nv); reference
- * scaling out_scale = d_pix^2/(du*dv). */
+ * scaling out_scale = d_pix^2/(du*dv).  VIR_COMPAT: the reference's sweep
+ * with its partially re-zeroed u buffer (dd.py:298-379), one thread per
+ * (view, bin) adding in the reference's line/slab order. */
 int vir_ddcone_fp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
                   const float *vol, float *sino, float out_scale, uint32_t flags,
                   void *stream);
 
-/* Exact transpose of vir_ddcone_fp; reference scaling 1/(na*d_pix). */
+/* Exact transpose of vir_ddcone_fp (of its default mode; VIR_COMPAT is
+ * ignored); reference scaling 1/(na*d_pix). */
 int vir_ddcone_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
                   const float *sino, float *vol, float out_scale, uint32_t flags,
                   void *stream);

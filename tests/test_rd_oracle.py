@@ -149,3 +149,37 @@ def test_reference_bp_scaling():
     a = rd_ref.bp(y, "par", img_shape=(8, 6), d_pix=0.8, reference_scale=True, **kw)
     b = rd_ref.bp(y, "par", img_shape=(8, 6), d_pix=0.8, reference_scale=False, **kw)
     np.testing.assert_allclose(a, b / 9 / 0.8 / 0.8 * 0.9, rtol=1e-12)
+
+
+def test_clamp_parallel_closed_form_and_transpose():
+    """clamp_parallel=True (rd.py:66-73): a parallel-beam ray along x that
+    passes outside the image is clamped onto the nearest border row, so its
+    AW sum is that row's sum times d; at 90 degrees the same holds for the
+    columns.  Inside rays are unchanged, and the clamped BP stays the
+    transpose of the clamped FP."""
+    rng = np.random.default_rng(21)
+    d, shape = 0.9, (6, 5)
+    img = rng.uniform(0.2, 1.0, shape)
+    ang = np.array([0.0, np.pi / 2])
+    nu, du, su = 17, 0.9, 0.3
+    fp0 = rd_ref.fp(img, "par", ang, nu, du, su, d)
+    fp1 = rd_ref.fp(img, "par", ang, nu, du, su, d, clamp_parallel=True)
+    u = du * (np.arange(nu) - (nu / 2 - 0.5) + su)
+    below, above = u < -shape[1] * d / 2, u >= shape[1] * d / 2
+    assert below.any() and above.any()
+    np.testing.assert_allclose(fp1[0, below], img[:, 0].sum() * d, rtol=1e-12)
+    np.testing.assert_allclose(fp1[0, above], img[:, -1].sum() * d, rtol=1e-12)
+    assert np.all(fp0[0, below | above] == 0)
+    inside = ~(below | above)
+    np.testing.assert_array_equal(fp1[0, inside], fp0[0, inside])
+    # 90 degrees: rays along y at x = -u
+    x = -u
+    left, right = x < -shape[0] * d / 2, x >= shape[0] * d / 2
+    assert left.any() and right.any()
+    np.testing.assert_allclose(fp1[1, left], img[0, :].sum() * d, rtol=1e-12)
+    np.testing.assert_allclose(fp1[1, right], img[-1, :].sum() * d, rtol=1e-12)
+    y = rng.uniform(0, 1, fp1.shape)
+    b = rd_ref.bp(y, "par", ang, shape, du, su, d, clamp_parallel=True, reference_scale=False)
+    x_ = rng.uniform(0, 1, shape)
+    f = rd_ref.fp(x_, "par", ang, nu, du, su, d, clamp_parallel=True)
+    assert abs(np.vdot(f, y) - np.vdot(x_, b)) <= 1e-12 * abs(np.vdot(f, y))

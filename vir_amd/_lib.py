@@ -17,6 +17,7 @@ VIR_SEQUENTIAL = 2
 VIR_UNGUARDED = 4
 VIR_PER_OUTPUT = 8
 VIR_SF_UNIT_AMPLITUDE = 16
+VIR_COMPAT = 32
 SIDDON_RECORD_BYTES = 80  # include/vir_amd.h: VIR_SIDDON_RECORD_BYTES
 
 P = ctypes.c_void_p

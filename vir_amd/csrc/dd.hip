@@ -369,6 +369,78 @@ __global__ __launch_bounds__(256) void k_ddcone_fp(DDGeo G, const float* __restr
     sino[t] = r;
 }
 
+// VIR_COMPAT cone forward: the reference's sweep (dd.py:298-379) with its
+// u buffer, one thread per (view, bin) owning sino[ia, iu, :].  For every
+// orthogonal line io and slab iz whose magnified z interval meets the
+// detector rows, the buffer entry of bin iu is recomputed when iu lies in
+// the line's re-zeroed range [iu_min, iu_max) (dd.py:313-317) and otherwise
+// keeps the value of the last line that recomputed it; the entry times each
+// row's v overlap is added to the sinogram in float32, in that order
+// (dd.py:366-379).  Lines wholly beyond the detector (iu_min >= nu) add
+// nothing.
+__global__ __launch_bounds__(256) void k_ddcone_fp_compat(DDGeo G, const float* __restrict__ cs,
+                                                         const float* __restrict__ sn,
+                                                         const float* __restrict__ vol, float* __restrict__ sino,
+                                                         int accumulate) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)G.na * G.nu) return;
+    const int iu = (int)(t % G.nu);
+    const int ia = (int)(t / G.nu);
+    const DDView V = dd_view(cs[ia], sn[ia]);
+    const int nd = V.xd ? G.nx : G.ny, no = V.xd ? G.ny : G.nx;
+    const float u0 = det_bnd(G.nu, G.du, G.su, 0), v0 = det_bnd(G.nv, G.dv, G.sv, 0);
+    const float ul = det_bnd(G.nu, G.du, G.su, iu), ur = det_bnd(G.nu, G.du, G.su, iu + 1);
+    const float tl = dd_slope<DD_FAN_FLAT>(G, ul), tr = dd_slope<DD_FAN_FLAT>(G, ur);
+    const float inv_d = 1.0f / G.d, inv_du = 1.0f / G.du, inv_dv = 1.0f / G.dv;
+    float* row = sino + t * G.nv;
+    if (!accumulate)
+        for (int iv = 0; iv < G.nv; ++iv) row[iv] = 0.0f;
+    float buf = 0.0f;
+    for (int io = 0; io < no; ++io) {
+        const float oc = cell_cen(no, G.d, io);
+        const float pa = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, 0), oc);
+        const float pz = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, nd), oc);
+        // int() truncation toward zero, arguments clamped to [-1, nu + 1]
+        const float fa = fminf(fmaxf((fminf(pa, pz) - u0) * inv_du, -1.0f), (float)G.nu + 1.0f);
+        const float fz = fminf(fmaxf((fmaxf(pa, pz) - u0) * inv_du, -1.0f), (float)G.nu + 1.0f);
+        const int iu_min = max((int)fa, 0), iu_max = min((int)fz + 1, G.nu);
+        if (iu_min >= G.nu) continue;
+        const bool refresh = iu >= iu_min && iu < iu_max;
+        const float zm = cone_zmag(G, V, oc);
+        const float sc = dd_scale<DD_FAN_FLAT>(G, V, oc);
+        // pixels of this line whose shadows can meet the bin
+        float x0 = dd_invert<DD_FAN_FLAT>(G, V, tl, oc), x1 = dd_invert<DD_FAN_FLAT>(G, V, tr, oc);
+        const float k0f = fminf(fmaxf(fminf(x0, x1) * inv_d + 0.5f * (float)nd, -4.0f), (float)nd + 4.0f);
+        const float k1f = fminf(fmaxf(fmaxf(x0, x1) * inv_d + 0.5f * (float)nd, -4.0f), (float)nd + 4.0f);
+        const int k0 = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
+        const int k1 = (int)fminf(floorf(k1f) + 1.0f, (float)(nd - 1));
+        for (int iz = 0; iz < G.nz; ++iz) {
+            const float zl = zm * cell_bnd(G.nz, G.d, iz), zr = zm * cell_bnd(G.nz, G.d, iz + 1);
+            const float ga = fminf(fmaxf((zl - v0) * inv_dv, -1.0f), (float)G.nv + 1.0f);
+            const float gz = fminf(fmaxf((zr - v0) * inv_dv, -1.0f), (float)G.nv + 1.0f);
+            const int iv_min = max((int)ga, 0), iv_max = min((int)gz + 1, G.nv);
+            if (iv_max <= 0 || iv_min >= G.nv) continue;
+            if (refresh) {
+                buf = 0.0f;
+                float pb = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, k0), oc);
+                for (int k = k0; k <= k1; ++k) {
+                    const float pn = dd_bound<DD_FAN_FLAT>(G, V, cell_bnd(nd, G.d, k + 1), oc);
+                    const float ovu = overlap(pb, pn, ul, ur);
+                    pb = pn;
+                    if (!(ovu > 0.0f)) continue;
+                    const int ix = V.xd ? k : io, iy = V.xd ? io : k;
+                    buf += (vol[((int64_t)ix * G.ny + iy) * G.nz + iz] * sc) * ovu;
+                }
+            }
+            if (buf == 0.0f) continue;
+            for (int iv = iv_min; iv < iv_max; ++iv) {
+                const float ov = fminf(zr, det_bnd(G.nv, G.dv, G.sv, iv + 1)) - fmaxf(zl, det_bnd(G.nv, G.dv, G.sv, iv));
+                if (ov > 0.0f) row[iv] += (buf * ov) * G.out_scale;
+            }
+        }
+    }
+}
+
 // One thread per voxel.
 __global__ __launch_bounds__(256) void k_ddcone_bp(DDGeo G, const float* __restrict__ cs,
                                                   const float* __restrict__ sn, const float* __restrict__ sino,
@@ -498,6 +570,12 @@ int vir_ddcone_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv,
     if (G.na == 0) return VIR_OK;
     if (!cosv || !sinv || !vol || !sino) return fail_arg("NULL pointer argument");
     G.out_scale = out_scale;
+    if (flags & VIR_COMPAT) {
+        const int64_t nb = (int64_t)G.na * G.nu;
+        hipLaunchKernelGGL(k_ddcone_fp_compat, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, as_stream(stream),
+                           G, cosv, sinv, vol, sino, (flags & VIR_ACCUMULATE) ? 1 : 0);
+        return check_hip(hipGetLastError(), "vir_ddcone_fp launch");
+    }
     int64_t n = (int64_t)G.na * G.nu * G.nv;
     hipLaunchKernelGGL(k_ddcone_fp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), G, cosv,
                        sinv, vol, sino, (flags & VIR_ACCUMULATE) ? 1 : 0);

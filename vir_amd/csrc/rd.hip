@@ -27,7 +27,9 @@
 //
 // Rays parallel to an axis (|direction| <= 1e-6) that lie outside the image
 // along that axis miss; the reference clamps their entry index onto the
-// border row instead (rd.py:66-73).
+// border row instead (rd.py:66-73), which VIR_COMPAT reproduces: such a ray
+// then hits, its index on that axis clamped, and the backprojection also
+// visits, for border voxels, the bins whose rays can be axis-parallel.
 #include <cmath>
 
 #include "vir_common.h"
@@ -44,6 +46,7 @@ struct RDGeo {
     float ucen;   // nu/2 - 0.5 in float32, the bin-centre offset
     float out_scale;
     float bmin[3], bmax[3], c0[3];
+    int compat;   // VIR_COMPAT: axis-parallel rays outside the image hit (clamped)
 };
 
 struct RDRay {
@@ -105,7 +108,7 @@ __device__ __forceinline__ RDRay rd_ray(const RDGeo& G, float c, float s, int iu
     }
     R.t0 = t0;
     R.t1 = t1;
-    R.hit = !miss && t1 > t0;
+    R.hit = (!miss || G.compat) && t1 > t0;
     return R;
 }
 
@@ -310,6 +313,45 @@ __device__ __forceinline__ float rd_joseph_w(const RDGeo& G, const RDRay& R, con
     return w * kRdStep;
 }
 
+// Bins [b0, b1] of view (c, s) whose rays can run parallel to axis a
+// (|h_a| <= kRdEps, rounding included); b1 < b0 when none can.  Parallel
+// beam: all bins or none.  Fan/cone: h_a = v_a/|v| with v_x = -DSD c - u s,
+// v_y = -DSD s + u c, so |v_a| <= W bounds u to an interval.
+template <int MODE>
+__device__ void rd_parallel_bins(const RDGeo& G, float c, float s, int a, int& b0, int& b1) {
+    b0 = 0;
+    b1 = -1;
+    if (MODE == RD_PAR2) {
+        if (fabsf(a == 0 ? c : s) <= kRdEps) b1 = G.nu - 1;
+        return;
+    }
+    const float k = a == 0 ? -s : c, q = a == 0 ? -G.DSD * c : -G.DSD * s;
+    const float umax = fmaxf(fabsf(rd_ucoord(G, 0)), fabsf(rd_ucoord(G, G.nu - 1)));
+    const float vmax = MODE == RD_CONE3 ? fmaxf(fabsf(rd_vcoord(G, 0)), fabsf(rd_vcoord(G, G.nv - 1))) : 0.0f;
+    const float W = 2e-6f * __builtin_sqrtf(G.DSD * G.DSD + umax * umax + vmax * vmax) + 1e-6f * (G.DSD + umax);
+    if (k == 0.0f) {
+        if (fabsf(q) <= W) b1 = G.nu - 1;
+        return;
+    }
+    const float ua = (-W - q) / k, ub = (W - q) / k;
+    const float fa = fminf(ua, ub) / G.du + G.ucen - G.su, fb = fmaxf(ua, ub) / G.du + G.ucen - G.su;
+    b0 = (int)fminf(fmaxf(floorf(fa) - 1.0f, 0.0f), (float)G.nu);
+    b1 = (int)fminf(fmaxf(ceilf(fb) + 1.0f, -1.0f), (float)(G.nu - 1));
+}
+
+// Adds ray (ia, iu, iv)'s weight on voxel vi times its sinogram value.
+template <int MODE, bool JOSEPH>
+__device__ __forceinline__ void rd_bp_bin(const RDGeo& G, const float* __restrict__ sino, float c, float s, int ia,
+                                          int iu, int iv, const int* vi, double& acc) {
+    const int nv = MODE == RD_CONE3 ? G.nv : 1;
+    const float y = sino[((int64_t)ia * G.nu + iu) * nv + iv];
+    if (y == 0.0f) return;
+    const RDRay R = rd_ray<MODE>(G, c, s, iu, iv);
+    if (!R.hit) return;
+    const float w = JOSEPH ? rd_joseph_w<MODE>(G, R, vi) : rd_aw_len<MODE>(G, R, vi);
+    if (w != 0.0f) acc += (double)(y * w);
+}
+
 template <int MODE, bool JOSEPH>
 __global__ __launch_bounds__(256) void k_rd_bp(RDGeo G, const float* __restrict__ cs, const float* __restrict__ sn,
                                               const float* __restrict__ sino, float* __restrict__ img,
@@ -333,21 +375,33 @@ __global__ __launch_bounds__(256) void k_rd_bp(RDGeo G, const float* __restrict_
         }
     }
     if (MODE != RD_CONE3) { lo[2] = 0.0f; hi[2] = 0.0f; }
+    // VIR_COMPAT, AW: a ray parallel to axis a outside the image is clamped
+    // onto voxel row 0 or n-1 of that axis without passing through it
+    const bool edge[2] = {G.compat && !JOSEPH && (vi[0] == 0 || vi[0] == G.nx - 1),
+                          G.compat && !JOSEPH && (vi[1] == 0 || vi[1] == G.ny - 1)};
     double acc = 0.0;
     for (int ia = 0; ia < G.na; ++ia) {
         const float c = cs[ia], s = sn[ia];
         int u0, u1, v0, v1;
         rd_bins<MODE>(G, c, s, lo, hi, u0, u1, v0, v1);
-        for (int iu = u0; iu <= u1; ++iu) {
-            for (int iv = v0; iv <= v1; ++iv) {
-                const float y = sino[((int64_t)ia * G.nu + iu) * nv + iv];
-                if (y == 0.0f) continue;
-                const RDRay R = rd_ray<MODE>(G, c, s, iu, iv);
-                if (!R.hit) continue;
-                const float w = JOSEPH ? rd_joseph_w<MODE>(G, R, vi) : rd_aw_len<MODE>(G, R, vi);
-                if (w != 0.0f) acc += (double)(y * w);
-            }
+        for (int iu = u0; iu <= u1; ++iu)
+            for (int iv = v0; iv <= v1; ++iv) rd_bp_bin<MODE, JOSEPH>(G, sino, c, s, ia, iu, iv, vi, acc);
+        if (!edge[0] && !edge[1]) continue;
+        int e0[2], e1[2];
+        for (int a = 0; a < 2; ++a) {
+            e0[a] = 0;
+            e1[a] = -1;
+            if (edge[a]) rd_parallel_bins<MODE>(G, c, s, a, e0[a], e1[a]);
         }
+        // every bin of the bands not visited above (nor by the band before)
+        for (int a = 0; a < 2; ++a)
+            for (int iu = e0[a]; iu <= e1[a]; ++iu) {
+                if (a == 1 && iu >= e0[0] && iu <= e1[0]) continue;
+                for (int iv = 0; iv < nv; ++iv) {
+                    if (iu >= u0 && iu <= u1 && iv >= v0 && iv <= v1) continue;
+                    rd_bp_bin<MODE, JOSEPH>(G, sino, c, s, ia, iu, iv, vi, acc);
+                }
+            }
     }
     float r = (float)acc * G.out_scale;
     if (accumulate) r += img[t];
@@ -362,6 +416,7 @@ static int rd_setup(const vir_rd_geom* g, RDGeo& G) {
     G.nu = g->nu; G.nv = is3 ? g->nv : 1; G.na = g->na;
     G.d = g->d_pix; G.du = g->du; G.dv = is3 ? g->dv : 1.0f; G.su = g->su; G.sv = is3 ? g->sv : 0.0f;
     G.DSO = g->DSO; G.DSD = g->DSD;
+    G.compat = 0;
     if (G.nx < 2 || G.ny < 2 || G.nz < 1 || (is3 && G.nz < 2) || G.nu < 1 || G.nv < 1 || G.na < 0)
         return fail_arg("bad sizes (image axes need >= 2 voxels)");
     if (!(G.d > 0.0f) || !(G.du > 0.0f) || !(G.dv > 0.0f)) return fail_arg("spacings must be > 0");
@@ -394,6 +449,7 @@ int vir_rd_fp(const vir_rd_geom* geom, const float* cosv, const float* sinv, con
     if (G.na == 0) return VIR_OK;
     if (!cosv || !sinv || !img || !sino) return fail_arg("NULL pointer argument");
     G.out_scale = out_scale;
+    G.compat = (flags & VIR_COMPAT) ? 1 : 0;
     const int64_t n = (int64_t)G.na * G.nu * G.nv;
     dim3 grd((unsigned)((n + 255) / 256));
     hipStream_t st = as_stream(stream);
@@ -418,6 +474,7 @@ int vir_rd_bp(const vir_rd_geom* geom, const float* cosv, const float* sinv, con
     if (rc) return rc;
     if ((G.na > 0 && (!cosv || !sinv || !sino)) || !img) return fail_arg("NULL pointer argument");
     G.out_scale = out_scale;
+    G.compat = (flags & VIR_COMPAT) ? 1 : 0;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     dim3 grd((unsigned)((n + 255) / 256));
     hipStream_t st = as_stream(stream);

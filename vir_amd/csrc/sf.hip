@@ -17,6 +17,11 @@
 // widened by the largest footprint half-width).  Backward: one thread per
 // (batch, pixel[, slab]) visits the bins its footprint meets.  Both evaluate
 // every weight with the same functions: exact transposes, no atomics.
+//
+// VIR_COMPAT (fan forward only): the reference skips a zero pixel without
+// advancing its o boundary (pd.py:321-325), so the next nonzero pixel's
+// amplitude takes its lower y boundary from just after the previous nonzero
+// pixel of the column (the first boundary if none); footprints are unchanged.
 #include <cmath>
 
 #include "vir_common.h"
@@ -31,6 +36,7 @@ struct SFGeo {
     float out_scale;
     float wmax;  // largest footprint half-width along u (detector units)
     int unit_amp;  // parallel modes: amplitude 1 (VIR_SF_UNIT_AMPLITUDE)
+    int compat;    // fan forward: stale o boundary after zero pixels (VIR_COMPAT)
 };
 
 __device__ __forceinline__ float sf_bnd(int n, float d, int k) { return d * ((float)k - 0.5f * (float)n); }
@@ -62,6 +68,21 @@ __device__ __forceinline__ SFView sf_view(const SFGeo& G, float c, float s, floa
     return V;
 }
 
+// Fan amplitude 1/(|cos(a+g)| + |sin(a+g)|) of pixel (ix, iy), g the angle
+// of the ray through its centre, whose o coordinate takes the pixel's lower
+// y boundary at boundary ky (ky = iy: the pixel's own).
+__device__ __forceinline__ float sf_fan_amp(const SFGeo& G, const SFView& V, int ix, int iy, int ky) {
+    const float c = V.c, s = V.s;
+    const float xl = sf_bnd(G.nx, G.d, ix), xr = sf_bnd(G.nx, G.d, ix + 1);
+    const float yl = sf_bnd(G.ny, G.d, iy), yr = sf_bnd(G.ny, G.d, iy + 1);
+    const float pxc = (-s * xl + -s * xr) * 0.5f, oxc = (c * xl + c * xr) * 0.5f;
+    const float pyc = (c * yl + c * yr) * 0.5f, oyc = (s * sf_bnd(G.ny, G.d, ky) + s * yr) * 0.5f;
+    const float uc = G.DSD * (pxc + pyc) / (G.DSO - (oxc + oyc));
+    const float gam = atanf(uc / G.DSD);
+    const float rx = cosf(V.ang + gam), ry = sinf(V.ang + gam);
+    return 1.0f / (fabsf(rx) + fabsf(ry));
+}
+
 // Footprint of pixel (ix, iy) in view V.  In parallel beam the four
 // projected corners -s*x + c*y span exactly [p_c - h, p_c + h] around the
 // centre's projection p_c, h = d(|sin| + |cos|)/2.
@@ -83,11 +104,6 @@ __device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, const SFView& V, int i
     const float yl = sf_bnd(G.ny, G.d, iy), yr = sf_bnd(G.ny, G.d, iy + 1);
     const float pxl = -s * xl, pxr = -s * xr, pyl = c * yl, pyr = c * yr;
     const float oxl = c * xl, oxr = c * xr, oyl = s * yl, oyr = s * yr;
-    const float pxc = (pxl + pxr) * 0.5f, oxc = (oxl + oxr) * 0.5f;
-    const float pyc = (pyl + pyr) * 0.5f, oyc = (oyl + oyr) * 0.5f;
-    const float den_c = G.DSO - (oxc + oyc);
-    const float uc = G.DSD * (pxc + pyc) / den_c;
-    const float gam = atanf(uc / G.DSD);
     if (MODE == SF_FAN2) {
         float p0 = G.DSD * (pxl + pyl) / (G.DSO - (oxl + oyl));
         float p1 = G.DSD * (pxr + pyl) / (G.DSO - (oxr + oyl));
@@ -95,11 +111,15 @@ __device__ __forceinline__ SFFoot sf_foot(const SFGeo& G, const SFView& V, int i
         float p3 = G.DSD * (pxr + pyr) / (G.DSO - (oxr + oyr));
         f.u0 = fminf(fminf(p0, p1), fminf(p2, p3));
         f.u1 = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
-        float rx = cosf(V.ang + gam), ry = sinf(V.ang + gam);
-        f.amp = 1.0f / (fabsf(rx) + fabsf(ry));
+        f.amp = sf_fan_amp(G, V, ix, iy, iy);
         return f;
     }
     // cone
+    const float pxc = (pxl + pxr) * 0.5f, oxc = (oxl + oxr) * 0.5f;
+    const float pyc = (pyl + pyr) * 0.5f, oyc = (oyl + oyr) * 0.5f;
+    const float den_c = G.DSO - (oxc + oyc);
+    const float uc = G.DSD * (pxc + pyc) / den_c;
+    const float gam = atanf(uc / G.DSD);
     const float fm = G.DSD / den_c;
     float u0 = fm * (pxl + pyl), u1 = fm * (pxr + pyl), u2 = fm * (pxl + pyr), u3 = fm * (pxr + pyr);
     f.u0 = fminf(fminf(u0, u1), fminf(u2, u3));
@@ -231,8 +251,14 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
             const SFFoot f = sf_foot<MODE>(G, V, ix, iy);
             const float ou = ovl(f.u0, f.u1, ul, ur);
             if (!(ou > 0.0f)) continue;
-            const float wu = (ou * inv_du) * f.amp;
+            float wu = (ou * inv_du) * f.amp;
             const float* col = vol + ((int64_t)ix * G.ny + iy) * nz;
+            if (MODE == SF_FAN2 && G.compat && col[0] != 0.0f) {
+                // the last nonzero pixel before iy in column ix
+                int j = iy - 1;
+                while (j >= 0 && vol[(int64_t)ix * G.ny + j] == 0.0f) --j;
+                if (j + 1 != iy) wu = (ou * inv_du) * sf_fan_amp(G, V, ix, iy, j + 1);
+            }
             if (!is3) {
                 acc += (double)(col[0] * wu);
             } else if (MODE == SF_PAR3 && !slab_general) {
@@ -740,6 +766,7 @@ static int sf_setup(const vir_sf_geom* g, SFGeo& G) {
     G.su = g->su; G.sv = is3 ? g->sv : 0.0f; G.DSO = g->DSO; G.DSD = g->DSD;
     G.out_scale = 1.0f;
     G.unit_amp = 0;
+    G.compat = 0;
     if (G.nx < 1 || G.ny < 1 || G.nz < 1 || G.nu < 1 || G.nv < 1 || G.na < 0 || G.nb < 1)
         return fail_arg("bad sizes");
     if (!(G.d > 0.0f) || !(G.du > 0.0f) || !(G.dv > 0.0f)) return fail_arg("spacings must be > 0");
@@ -788,6 +815,7 @@ int vir_sf_fp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
     if (!ang || !cosv || !sinv || !img || !sino) return fail_arg("NULL pointer argument");
     G.out_scale = out_scale;
     G.unit_amp = (flags & VIR_SF_UNIT_AMPLITUDE) ? 1 : 0;
+    G.compat = (flags & VIR_COMPAT) ? 1 : 0;
     int64_t n = (int64_t)G.nb * G.na * G.nu * G.nv;
     dim3 grd((unsigned)((n + 255) / 256));
     hipStream_t st = as_stream(stream);

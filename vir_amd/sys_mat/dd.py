@@ -54,8 +54,14 @@ class DDProjector:
     (np.cos of a float32 array)."""
 
     def __init__(self, img_shape, ang_arr, nu, geom="par", nv=1, DSO=0.0, DSD=0.0, du=1.0,
-                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, detector="flat"):
+                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, detector="flat", compat=False):
+        """compat (cone): the forward pass re-zeroes its u buffer only over
+        each line's footprint, as the reference's sweep does (dd.py:317, 374);
+        the backward pass is unchanged."""
         require_gpu()
+        if compat and geom != "cone":
+            raise ValueError("compat applies to the cone geometry")
+        self._fp_flags = _lib.VIR_COMPAT if compat else 0
         ang = np.asarray(ang_arr, dtype=np.float32).reshape(-1)
         self.na = ang.size
         self.geom = geom
@@ -90,6 +96,8 @@ class DDProjector:
 
     def _call(self, name, src, dst, scale, accumulate):
         fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        if name == "vir_ddcone_fp":
+            fl |= self._fp_flags
         _lib.check(getattr(_lib.lib(), name)(self.g, _lib.ptr(self.cos), _lib.ptr(self.sin),
                                              _lib.ptr(src), _lib.ptr(dst), scale, fl, stream()), name)
 
@@ -158,11 +166,13 @@ def dd_bp_fan_2d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, 
 
 
 def dd_fp_cone_3d(img, ang_arr, nu, nv, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=1.0, d_pix=1.0,
-                  sample=1):
+                  sample=1, compat=False):
     """Cone DD forward projection.  The u buffer is cleared fully per slab;
-    the reference clears only the current footprint (dd.py:317), which adds
-    leftovers when the object touches the volume border (identical results
-    otherwise)."""
+    the reference clears only the current footprint (dd.py:317) and adds the
+    whole buffer (dd.py:374), which carries leftovers of earlier lines into
+    bins outside the current footprint when the object touches the volume
+    border (identical results otherwise).  compat=True reproduces the
+    reference."""
     torch_in = isinstance(img, torch.Tensor)
     img, ang_arr, DSO, DSD, du, dv, su, sv, d_pix = as_float32(img, ang_arr, DSO, DSD, du, dv, su,
                                                                sv, d_pix)
@@ -170,7 +180,7 @@ def dd_fp_cone_3d(img, ang_arr, nu, nv, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=1.0
         img = _kron_sample(img, sample)
         d_pix = np.float32(d_pix / sample)
     P = DDProjector(tuple(img.shape), ang_arr, nu, "cone", nv=nv, DSO=DSO, DSD=DSD, du=du, dv=dv,
-                    su=su, sv=sv, d_pix=d_pix)
+                    su=su, sv=sv, d_pix=d_pix, compat=compat)
     return _wrap(P.fp(img), torch_in)
 
 

@@ -24,7 +24,8 @@ The fan forward pass evaluates every pixel's footprint from its own
 boundaries.  The reference's loop skips zero pixels without advancing one
 of the boundaries it uses for the next pixel's amplitude (pd.py:321-325),
 so its results differ on images with zero runs in a column; for strictly
-nonzero images the two agree.
+nonzero images the two agree.  pd_fp_fan_2d(..., compat=True) (and
+SFProjector(geom="fan", compat=True).fp) reproduces the reference's loop.
 
 numpy in -> numpy out; CUDA tensors in -> CUDA tensors out.
 """
@@ -45,13 +46,20 @@ class SFProjector:
     > 1).  Angles in radians; cos/sin are rounded to float32."""
 
     def __init__(self, img_shape, ang_arr, nu, geom="par", nv=1, DSO=0.0, DSD=0.0, du=1.0,
-                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, batch=1, unit_amplitude=False):
+                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, batch=1, unit_amplitude=False,
+                 compat=False):
         """unit_amplitude (parallel geometries): footprint amplitude 1 instead
-        of 1/(|sin| + |cos|), the rectangle footprint of fp_2d_par.py:67-125."""
+        of 1/(|sin| + |cos|), the rectangle footprint of fp_2d_par.py:67-125.
+        compat (fan): the forward pass takes the amplitude boundary after a
+        run of zero pixels as the reference's loop does (pd.py:321-325); the
+        backward pass is unchanged (the reference's has no such skip)."""
         require_gpu()
         if unit_amplitude and geom not in ("par", "par3d"):
             raise ValueError("unit_amplitude applies to the parallel geometries")
+        if compat and geom != "fan":
+            raise ValueError("compat applies to the fan geometry")
         self._flags = _lib.VIR_SF_UNIT_AMPLITUDE if unit_amplitude else 0
+        self._fp_flags = _lib.VIR_COMPAT if compat else 0
         if geom not in _MODES:
             raise ValueError(f"unknown geometry {geom!r}")
         ang64 = np.asarray(ang_arr, dtype=np.float64).reshape(-1)
@@ -84,6 +92,8 @@ class SFProjector:
 
     def _call(self, name, src, dst, scale, accumulate):
         fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._flags
+        if name == "vir_sf_fp":
+            fl |= self._fp_flags
         _lib.check(getattr(_lib.lib(), name)(self.g, _lib.ptr(self.ang), _lib.ptr(self.cos),
                                              _lib.ptr(self.sin), _lib.ptr(src), _lib.ptr(dst),
                                              scale, fl, stream()), name)
@@ -124,10 +134,10 @@ def pd_bp_par_2d(sino, ang_arr, img_shape, du=1.0, su=0.0, d_pix=1.0):
     return _wrap(P.bp(sino, reference_scale=True), torch_in)
 
 
-def pd_fp_fan_2d(img, ang_arr, nu, DSO, DSD, du=1.0, su=0.0, d_pix=1.0):
+def pd_fp_fan_2d(img, ang_arr, nu, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, compat=False):
     torch_in = isinstance(img, torch.Tensor)
     P = SFProjector(tuple(img.shape), ang_arr, nu, "fan", DSO=DSO, DSD=DSD, du=du, su=su,
-                    d_pix=d_pix)
+                    d_pix=d_pix, compat=compat)
     return _wrap(P.fp(img), torch_in)
 
 

@@ -23,8 +23,10 @@ image border (the 2-D forward clamps samples, the backward splats with
 bounds checks; the 3-D forward truncates indices toward zero).
 
 Rays parallel to an axis that lie outside the image miss here; the
-reference clamps them onto the border row (rd.py:66-73).  Cone rows are
-centred with nu as in the reference (rd.py:691).
+reference clamps them onto the border row (rd.py:66-73), so at views along
+an axis with a detector wider than the image its outer bins repeat the
+border row's sums.  compat=True reproduces that (forward and backward).
+Cone rows are centred with nu as in the reference (rd.py:691).
 
 numpy in -> numpy out; CUDA tensors in -> CUDA tensors out.
 """
@@ -41,8 +43,11 @@ class RDProjector:
     """Ray-driven forward projector and the reference's backprojector."""
 
     def __init__(self, img_shape, ang_arr, nu, geom="par", nv=1, DSO=0.0, DSD=0.0, du=1.0,
-                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, joseph=False):
+                 dv=1.0, su=0.0, sv=0.0, d_pix=1.0, joseph=False, compat=False):
+        """compat=True: rays parallel to an axis outside the image are
+        clamped onto the border row as in the reference (rd.py:66-73)."""
         require_gpu()
+        self._flags = _lib.VIR_COMPAT if compat else 0
         if geom not in _MODES:
             raise ValueError(f"unknown geometry {geom!r}")
         ang = np.asarray(ang_arr, dtype=np.float32).reshape(-1)
@@ -72,7 +77,7 @@ class RDProjector:
             self.ref_bp_scale = float(du32 / (na * d * d))
 
     def _call(self, name, src, dst, scale, accumulate):
-        fl = _lib.VIR_ACCUMULATE if accumulate else 0
+        fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._flags
         _lib.check(getattr(_lib.lib(), name)(self.g, _lib.ptr(self.cos), _lib.ptr(self.sin),
                                              _lib.ptr(src), _lib.ptr(dst), scale, fl, stream()), name)
 
@@ -100,44 +105,46 @@ def _wrap(x, torch_in):
     return x if torch_in else x.cpu().numpy()
 
 
-def aw_fp_par_2d(img, ang_arr, nu, du=1.0, su=0.0, d_pix=1.0, joseph=False):
+def aw_fp_par_2d(img, ang_arr, nu, du=1.0, su=0.0, d_pix=1.0, joseph=False, compat=False):
     torch_in = isinstance(img, torch.Tensor)
-    P = RDProjector(tuple(img.shape), ang_arr, nu, "par", du=du, su=su, d_pix=d_pix, joseph=joseph)
+    P = RDProjector(tuple(img.shape), ang_arr, nu, "par", du=du, su=su, d_pix=d_pix,
+                    joseph=joseph, compat=compat)
     return _wrap(P.fp(img), torch_in)
 
 
-def aw_bp_par_2d(sino, ang_arr, img_shape, du=1.0, su=0.0, d_pix=1.0, joseph=False):
+def aw_bp_par_2d(sino, ang_arr, img_shape, du=1.0, su=0.0, d_pix=1.0, joseph=False, compat=False):
     torch_in = isinstance(sino, torch.Tensor)
     P = RDProjector(img_shape, ang_arr, sino.shape[1], "par", du=du, su=su, d_pix=d_pix,
-                    joseph=joseph)
+                    joseph=joseph, compat=compat)
     return _wrap(P.bp(sino, reference_scale=True), torch_in)
 
 
-def aw_fp_fan_2d(img, ang_arr, nu, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, joseph=False):
+def aw_fp_fan_2d(img, ang_arr, nu, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, joseph=False, compat=False):
     torch_in = isinstance(img, torch.Tensor)
     P = RDProjector(tuple(img.shape), ang_arr, nu, "fan", DSO=DSO, DSD=DSD, du=du, su=su,
-                    d_pix=d_pix, joseph=joseph)
+                    d_pix=d_pix, joseph=joseph, compat=compat)
     return _wrap(P.fp(img), torch_in)
 
 
-def aw_bp_fan_2d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, joseph=False):
+def aw_bp_fan_2d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, su=0.0, d_pix=1.0, joseph=False,
+                 compat=False):
     torch_in = isinstance(sino, torch.Tensor)
     P = RDProjector(img_shape, ang_arr, sino.shape[1], "fan", DSO=DSO, DSD=DSD, du=du, su=su,
-                    d_pix=d_pix, joseph=joseph)
+                    d_pix=d_pix, joseph=joseph, compat=compat)
     return _wrap(P.bp(sino, reference_scale=True), torch_in)
 
 
 def aw_fp_cone_3d(img, ang_arr, nu, nv, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=0.0, d_pix=1.0,
-                  joseph=False):
+                  joseph=False, compat=False):
     torch_in = isinstance(img, torch.Tensor)
     P = RDProjector(tuple(img.shape), ang_arr, nu, "cone", nv=nv, DSO=DSO, DSD=DSD, du=du, dv=dv,
-                    su=su, sv=sv, d_pix=d_pix, joseph=joseph)
+                    su=su, sv=sv, d_pix=d_pix, joseph=joseph, compat=compat)
     return _wrap(P.fp(img), torch_in)
 
 
 def aw_bp_cone_3d(sino, ang_arr, img_shape, DSO, DSD, du=1.0, dv=1.0, su=0.0, sv=0.0, d_pix=1.0,
-                  joseph=False):
+                  joseph=False, compat=False):
     torch_in = isinstance(sino, torch.Tensor)
     P = RDProjector(img_shape, ang_arr, sino.shape[1], "cone", nv=sino.shape[2], DSO=DSO,
-                    DSD=DSD, du=du, dv=dv, su=su, sv=sv, d_pix=d_pix, joseph=joseph)
+                    DSD=DSD, du=du, dv=dv, su=su, sv=sv, d_pix=d_pix, joseph=joseph, compat=compat)
     return _wrap(P.bp(sino, reference_scale=True), torch_in)
