@@ -95,6 +95,27 @@ def test_waiting_request_does_not_overtake():
     assert res[1].queued > 0.015   # waited for request 0 to finish
 
 
+class KeyedBackend(FakeBackend):
+    @staticmethod
+    def geometry_key(req):
+        return (req.kind, req.n, req.views)
+
+
+def test_grouped_admission_pulls_same_geometry_forward():
+    reqs = [Request(0, "fan", 4, 3), Request(1, "fan", 2, 1), Request(2, "fan", 8, 2),
+            Request(3, "fan", 4, 3), Request(4, "fan", 2, 1)]
+    be = KeyedBackend()
+    srv = Server(be, hbm_budget=100_000, max_active=1, group=True)
+    res = srv.run(reqs)
+    assert all(r.ok for r in res)
+    assert [e[1] for e in be.log if e[0] == "start"] == [0, 3, 1, 4, 2]
+    assert srv.grouped == 2
+    # without grouping: arrival order
+    be = KeyedBackend()
+    Server(be, hbm_budget=100_000, max_active=1).run(reqs)
+    assert [e[1] for e in be.log if e[0] == "start"] == [0, 1, 2, 3, 4]
+
+
 def test_poisson_trace_mix():
     tr = poisson_trace(4000, 200.0, seed=5)
     kinds = np.array([r.kind == "fan" for r in tr])

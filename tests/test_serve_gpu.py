@@ -158,3 +158,26 @@ def test_request_over_budget_is_rejected_and_others_run():
     budget = be.estimate(small) * 2
     res = Server(be, hbm_budget=budget, max_active=2).run([small, big])
     assert res[0].ok and not res[1].ok and "budget" in res[1].error
+
+
+def test_cached_and_grouped_geometries_give_the_alone_results():
+    """Requests of one geometry share the cached projector and W sums and
+    run on one stream; their results are bitwise those of requests run
+    alone (each building its own projector)."""
+    from vir_amd.serve import Request, Server, SirtBackend, run_alone
+    reqs = [Request(0, "fan", 48, 30, iters=4, seed=1), Request(1, "cone", 24, 20, iters=3, seed=2),
+            Request(2, "fan", 48, 30, iters=4, seed=3), Request(3, "cone", 24, 20, iters=3, seed=4),
+            Request(4, "fan", 32, 20, iters=2, seed=5)]
+    be = SirtBackend(n_streams=2, cache_bytes=1 << 30)
+    srv = Server(be, hbm_budget=8 << 30, max_active=4, group=True)
+    res = srv.run(reqs)
+    assert all(r.ok for r in res), [r.error for r in res if not r.ok]
+    assert (be.cache_hits, be.cache_misses) == (2, 3)
+    for r, q in zip(res, reqs):
+        assert np.array_equal(r.x, run_alone(q)), q
+    # a zero-byte cache keeps nothing
+    be0 = SirtBackend(n_streams=1, cache_bytes=0)
+    res0 = Server(be0, hbm_budget=8 << 30, max_active=2).run(reqs[:3])
+    assert all(r.ok for r in res0) and be0.cache_hits == 0 and not be0._cache
+    for r, q in zip(res0, reqs[:3]):
+        assert np.array_equal(r.x, run_alone(q)), q

@@ -21,6 +21,13 @@ slot free up (continuous batching at request granularity).  Every kernel on
 the path is deterministic, so a request's result does not depend on what
 else is running.
 
+Requests of the same geometry (kind, size, views) share one projector:
+the backend keeps the projector and its normalisation sums W_sino, W_img in
+an LRU cache bounded in bytes, and the server (group=True) admits waiting
+requests of the geometry it has just admitted right after it.  Jobs that
+share a projector run on one stream (the projector's backprojection
+workspace is per projector).
+
 Fan-beam requests use the distance-driven projector with an equiangular
 detector (sys_mat/dd.py, the reference's fan geometry on an arc); cone-beam
 requests use the on-the-fly Siddon projector with the brick-gather
@@ -101,8 +108,12 @@ class Server:
     handle.release().  Slots index the backend's streams."""
 
     def __init__(self, backend, hbm_budget, max_active=16, clock=time.perf_counter,
-                 idle_sleep=2e-4):
+                 idle_sleep=2e-4, group=False):
+        """group: after admitting a request, admit the waiting requests
+        with the same backend.geometry_key next (out of arrival order)."""
         self.backend = backend
+        self.group_key = getattr(backend, "geometry_key", None) if group else None
+        self.grouped = 0
         self.budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.clock = clock
@@ -165,6 +176,13 @@ class Server:
                 self.max_in_use = max(self.max_in_use, self.in_use)
                 self.max_concurrent = max(self.max_concurrent, len(active))
                 progressed = True
+                if self.group_key is not None and waiting:
+                    k = self.group_key(req)
+                    same = [w for w in waiting if self.group_key(w) == k]
+                    for w in same:
+                        waiting.remove(w)
+                    waiting.extendleft(reversed(same))
+                    self.grouped += len(same)
             if not progressed:
                 if not active and not waiting and pending:
                     dt = pending[0].arrival - (self.clock() - t0)
@@ -208,22 +226,21 @@ def poisson_trace(n_requests, rate, seed=0, iters=20, fan_frac=0.6,
 
 # ------------------------------------------------------- the GPU backend
 
-class _Job:
-    def __init__(self, req, stream, graph=True):
+class _Geometry:
+    """A request geometry's projector and normalisation sums, shared by the
+    jobs of that geometry."""
+
+    def __init__(self, req, stream):
         import torch
         from .geometry import circular_projections, circular_rays
-        from .sirt import SIRT
         from .sys_mat.dd import DDProjector
         from .sys_mat.siddon import SiddonProjector
-        from .workloads import phantom_device
-        self.req = req
-        self.stream = stream
         with torch.cuda.stream(stream):
             if req.kind == "fan":
                 g = fan_geometry(req.n, req.views)
                 P = DDProjector(g["img_shape"], g["ang"], g["nu"], "fan", DSO=g["DSO"],
                                 DSD=g["DSD"], du=g["du"], detector="arc")
-                n_ell = 10
+                nbytes = 8 * req.views
             elif req.kind == "cone":
                 g = cone_geometry(req.n, req.views)
                 src, trg = circular_rays(g["dets_y"], g["views"], "cone", g["src_iso"],
@@ -231,13 +248,36 @@ class _Job:
                 mats = circular_projections(g["dets_y"], g["views"], "cone", g["src_iso"],
                                             g["det_iso"], g["dets_z"])
                 P = SiddonProjector(src, trg, g["img_shape"], 1.0, proj=mats)
-                n_ell = 20
+                nbytes = P.device_bytes()
             else:
                 raise ValueError(f"unknown request kind {req.kind!r}")
+            img_shape, sino_shape = tuple(g["img_shape"]), tuple(P.sino_shape if req.kind == "fan"
+                                                                 else P.ray_shape)
+            self.W_sino = P.fp(torch.ones(img_shape, dtype=torch.float32, device="cuda"))
+            self.W_img = P.bp(torch.ones(sino_shape, dtype=torch.float32, device="cuda"))
+        self.P, self.g = P, g
+        self.nbytes = nbytes + 4 * (self.W_sino.numel() + self.W_img.numel())
+        self.stream = stream
+        self.users = 0
+
+
+class _Job:
+    def __init__(self, req, stream, graph=True, geometry=None):
+        import torch
+        from .sirt import SIRT
+        from .workloads import phantom_device
+        self.req = req
+        self.stream = stream
+        G = geometry if geometry is not None else _Geometry(req, stream)
+        G.users += 1
+        self.geometry = G
+        with torch.cuda.stream(stream):
+            P, g = G.P, G.g
+            n_ell = 10 if req.kind == "fan" else 20
             truth = phantom_device(g["img_shape"], n_ell, req.seed, stream=stream)
             b = P.fp(truth)            # the measurement, synthesised on the device
             del truth
-            S = SIRT(P, b, gamma=req.gamma)
+            S = SIRT(P, b, gamma=req.gamma, W_sino=G.W_sino, W_img=G.W_img)
             self.P, self.S = P, S
             self.host = torch.empty(S.img_shape, dtype=torch.float32, pin_memory=True)
             if graph:
@@ -270,17 +310,32 @@ class _Job:
     def release(self):
         self.graph = None
         self.P = self.S = None
+        if self.geometry is not None:
+            self.geometry.users -= 1
+            self.geometry = None
 
 
 class SirtBackend:
-    """Runs requests as device-resident, graph-captured SIRT jobs."""
+    """Runs requests as device-resident, graph-captured SIRT jobs.
 
-    def __init__(self, n_streams=8, graph=True):
+    cache_bytes: device bytes of projectors (with W_sino, W_img) kept for
+    later requests of the same geometry (least recently used evicted first,
+    never while a job uses it); 0 disables the cache."""
+
+    def __init__(self, n_streams=8, graph=True, cache_bytes=32 << 30):
         import torch
         from .device import require_gpu
         require_gpu()
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
         self.graph = graph
+        self.cache_bytes = int(cache_bytes)
+        self._cache = collections.OrderedDict()
+        self.cache_hits = 0
+        self.cache_misses = 0
+
+    @staticmethod
+    def geometry_key(req):
+        return (req.kind, req.n, req.views)
 
     @staticmethod
     def estimate(req):
@@ -303,8 +358,51 @@ class SirtBackend:
         # allocator granularity: every tensor rounded up to 2 MiB at most
         return int(b * 1.02) + 24 * (2 << 20)
 
+    def _evict(self):
+        total = sum(e.nbytes for e in self._cache.values())
+        for k in list(self._cache):
+            if total <= self.cache_bytes:
+                break
+            e = self._cache[k]
+            if e.users == 0:
+                del self._cache[k]
+                total -= e.nbytes
+
     def start(self, req, slot):
-        return _Job(req, self.streams[slot % len(self.streams)], graph=self.graph)
+        key = self.geometry_key(req)
+        G = self._cache.get(key)
+        if G is not None:
+            self._cache.move_to_end(key)
+            self.cache_hits += 1
+            # jobs sharing a projector share its workspace: same stream
+            stream = G.stream if G.users > 0 else self.streams[slot % len(self.streams)]
+            G.stream = stream
+        else:
+            self.cache_misses += 1
+            stream = self.streams[slot % len(self.streams)]
+            G = _Geometry(req, stream)
+            if 0 < G.nbytes <= self.cache_bytes:
+                self._cache[key] = G
+        job = _Job(req, stream, graph=self.graph, geometry=G)
+        self._evict()
+        return _Released(job, self)
+
+
+class _Released:
+    """A job handle whose release also trims the backend's cache."""
+
+    def __init__(self, job, backend):
+        self.job, self.backend = job, backend
+
+    def poll(self):
+        return self.job.poll()
+
+    def result(self):
+        return self.job.result()
+
+    def release(self):
+        self.job.release()
+        self.backend._evict()
 
 
 def run_alone(req, graph=True):
