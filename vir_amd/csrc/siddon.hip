@@ -188,6 +188,84 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
     w = P[8] * x + P[9] * y + P[10] * z + P[11];
 }
 
+// Footprint of the guard box [gb_lo, gb_hi] in view P on the detector,
+// computed by the whole wave (every lane must call it).
+//
+// Candidate rays: a ray with an entry in the sub-brick meets the guard
+// box, so its index lies in the box's projected range (the convex hull of
+// the projected corners, all in front of the source) widened by the
+// matrix fit error (hosts accept matrices within 0.05 index units).
+//
+// Colour strides: for two points Y, Z of one voxel, with rows a (u),
+// c (w) of the matrix and u = a.Y~ / c.Y~,
+//   u(Z) - u(Y) = sum_i (Z_i - Y_i) (a_i - u(Y) c_i) / c.Z~,
+// so every voxel of the guard box projects to a u-extent of at most
+//   fu = sum_i d_i max(|a_i - umin c_i|, |a_i - umax c_i|) / wmin
+// (|a_i - u c_i| is convex in u, so its maximum over the box's range
+// [umin, umax] is at an end; c.X~ is affine, so its minimum wmin is at
+// a corner).  Two rays through one voxel are at most fu + 2 * 0.05 apart
+// in u, so rays whose u-indices differ by su > 1.05 fu + 0.15 never
+// share a voxel; likewise in v.  The 5 % cover float rounding and
+// entries whose computed index is one voxel off their true midpoint by
+// ~1e-3 voxel.
+__device__ __forceinline__ ViewFoot view_foot(const GridK& g, const float* __restrict__ P, const float* gb_lo,
+                                              const float* gb_hi, int nu, int nv, int lane) {
+    float gu = 0.f, gv = 0.f, gw = 0.f;
+    project(P, (lane & 4) ? gb_hi[0] : gb_lo[0], (lane & 2) ? gb_hi[1] : gb_lo[1],
+            (lane & 1) ? gb_hi[2] : gb_lo[2], gu, gv, gw);
+    int gok = (lane >= 8) || (gw > 0.0f);
+    float guu = lane < 8 ? gu / gw : VIR_INF, gvv = lane < 8 ? gv / gw : VIR_INF;
+    float gumin = guu, gvmin = gvv;
+    float gumax = lane < 8 ? guu : -VIR_INF, gvmax = lane < 8 ? gvv : -VIR_INF;
+    float gwmin = lane < 8 ? gw : VIR_INF;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        gumin = fminf(gumin, __shfl_xor(gumin, m));
+        gvmin = fminf(gvmin, __shfl_xor(gvmin, m));
+        gumax = fmaxf(gumax, __shfl_xor(gumax, m));
+        gvmax = fmaxf(gvmax, __shfl_xor(gvmax, m));
+        gwmin = fminf(gwmin, __shfl_xor(gwmin, m));
+        gok &= __shfl_xor(gok, m);
+    }
+    // lane 0 holds the wave's results; make them wave-uniform
+    gumin = __shfl(gumin, 0); gumax = __shfl(gumax, 0);
+    gvmin = __shfl(gvmin, 0); gvmax = __shfl(gvmax, 0);
+    gwmin = __shfl(gwmin, 0);
+    gok = __shfl(gok, 0);
+    float ku = 0.0f, kv = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float c = P[8 + i];
+        ku = ku + g.d[i] * fmaxf(fabsf(P[i] - gumin * c), fabsf(P[i] - gumax * c));
+        kv = kv + g.d[i] * fmaxf(fabsf(P[4 + i] - gvmin * c), fabsf(P[4 + i] - gvmax * c));
+    }
+    const float fu = ku / gwmin, fv = kv / gwmin;
+
+    ViewFoot f;
+    const float range_margin = 0.075f, stride_margin = 0.15f;
+    if (!gok || !(fu < 1.0e6f) || !(fv < 1.0e6f)) {
+        // degenerate projection (hosts validate geometry, so not taken)
+        f.iu0 = 0; f.iu1 = nu - 1; f.iv0 = 0; f.iv1 = nv - 1;
+        f.su = nu; f.sv = nv;
+    } else {
+        float a0 = ceilf(gumin - range_margin), a1 = floorf(gumax + range_margin);
+        float c0 = ceilf(gvmin - range_margin), c1 = floorf(gvmax + range_margin);
+        f.iu0 = (int)fmaxf(a0, 0.0f);
+        f.iu1 = (int)fminf(a1, (float)(nu - 1));
+        f.iv0 = (int)fmaxf(c0, 0.0f);
+        f.iv1 = (int)fminf(c1, (float)(nv - 1));
+        if (a1 < 0.0f || a0 > (float)(nu - 1)) f.iu1 = f.iu0 - 1;
+        if (c1 < 0.0f || c0 > (float)(nv - 1)) f.iv1 = f.iv0 - 1;
+        f.su = (int)floorf(fu * 1.05f + stride_margin) + 1;
+        f.sv = nv == 1 ? 1 : (int)floorf(fv * 1.05f + stride_margin) + 1;
+        // a stride wider than the footprint already puts every column
+        // (row) in its own phase
+        f.su = min(f.su, max(f.iu1 - f.iu0 + 1, 1));
+        f.sv = min(f.sv, max(f.iv1 - f.iv0 + 1, 1));
+    }
+    return f;
+}
+
 // Each wave owns one sub-brick of SBX x SBY x SBZ voxels with a private
 // on-chip accumulator; a workgroup of four waves covers a 2x2 block of
 // sub-bricks (neighbouring sub-bricks share rays through L1/L2).  Per view a
@@ -242,81 +320,8 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
     const int a_begin = blockIdx.y * views_per_chunk;
     const int a_end = live ? min(nviews, a_begin + views_per_chunk) : a_begin;
 
-    // footprint of view va on the detector, computed by the whole wave.
-    //
-    // Candidate rays: a ray with an entry in the sub-brick meets the guard
-    // box, so its index lies in the box's projected range (the convex hull of
-    // the projected corners, all in front of the source) widened by the
-    // matrix fit error (hosts accept matrices within 0.05 index units).
-    //
-    // Colour strides: for two points Y, Z of one voxel, with rows a (u),
-    // c (w) of the matrix and u = a.Y~ / c.Y~,
-    //   u(Z) - u(Y) = sum_i (Z_i - Y_i) (a_i - u(Y) c_i) / c.Z~,
-    // so every voxel of the guard box projects to a u-extent of at most
-    //   fu = sum_i d_i max(|a_i - umin c_i|, |a_i - umax c_i|) / wmin
-    // (|a_i - u c_i| is convex in u, so its maximum over the box's range
-    // [umin, umax] is at an end; c.X~ is affine, so its minimum wmin is at
-    // a corner).  Two rays through one voxel are at most fu + 2 * 0.05 apart
-    // in u, so rays whose u-indices differ by su > 1.05 fu + 0.15 never
-    // share a voxel; likewise in v.  The 5 % cover float rounding and
-    // entries whose computed index is one voxel off their true midpoint by
-    // ~1e-3 voxel.
     auto view_foot = [&](int va) -> ViewFoot {
-        const float* P = proj + 12 * va;
-        float gu = 0.f, gv = 0.f, gw = 0.f;
-        project(P, (lane & 4) ? gb_hi[0] : gb_lo[0], (lane & 2) ? gb_hi[1] : gb_lo[1],
-                (lane & 1) ? gb_hi[2] : gb_lo[2], gu, gv, gw);
-        int gok = (lane >= 8) || (gw > 0.0f);
-        float guu = lane < 8 ? gu / gw : VIR_INF, gvv = lane < 8 ? gv / gw : VIR_INF;
-        float gumin = guu, gvmin = gvv;
-        float gumax = lane < 8 ? guu : -VIR_INF, gvmax = lane < 8 ? gvv : -VIR_INF;
-        float gwmin = lane < 8 ? gw : VIR_INF;
-#pragma unroll
-        for (int m = 1; m < 8; m <<= 1) {
-            gumin = fminf(gumin, __shfl_xor(gumin, m));
-            gvmin = fminf(gvmin, __shfl_xor(gvmin, m));
-            gumax = fmaxf(gumax, __shfl_xor(gumax, m));
-            gvmax = fmaxf(gvmax, __shfl_xor(gvmax, m));
-            gwmin = fminf(gwmin, __shfl_xor(gwmin, m));
-            gok &= __shfl_xor(gok, m);
-        }
-        // lane 0 holds the wave's results; make them wave-uniform
-        gumin = __shfl(gumin, 0); gumax = __shfl(gumax, 0);
-        gvmin = __shfl(gvmin, 0); gvmax = __shfl(gvmax, 0);
-        gwmin = __shfl(gwmin, 0);
-        gok = __shfl(gok, 0);
-        float ku = 0.0f, kv = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const float c = P[8 + i];
-            ku = ku + g.d[i] * fmaxf(fabsf(P[i] - gumin * c), fabsf(P[i] - gumax * c));
-            kv = kv + g.d[i] * fmaxf(fabsf(P[4 + i] - gvmin * c), fabsf(P[4 + i] - gvmax * c));
-        }
-        const float fu = ku / gwmin, fv = kv / gwmin;
-
-        ViewFoot f;
-        const float range_margin = 0.075f, stride_margin = 0.15f;
-        if (!gok || !(fu < 1.0e6f) || !(fv < 1.0e6f)) {
-            // degenerate projection (hosts validate geometry, so not taken)
-            f.iu0 = 0; f.iu1 = nu - 1; f.iv0 = 0; f.iv1 = nv - 1;
-            f.su = nu; f.sv = nv;
-        } else {
-            float a0 = ceilf(gumin - range_margin), a1 = floorf(gumax + range_margin);
-            float c0 = ceilf(gvmin - range_margin), c1 = floorf(gvmax + range_margin);
-            f.iu0 = (int)fmaxf(a0, 0.0f);
-            f.iu1 = (int)fminf(a1, (float)(nu - 1));
-            f.iv0 = (int)fmaxf(c0, 0.0f);
-            f.iv1 = (int)fminf(c1, (float)(nv - 1));
-            if (a1 < 0.0f || a0 > (float)(nu - 1)) f.iu1 = f.iu0 - 1;
-            if (c1 < 0.0f || c0 > (float)(nv - 1)) f.iv1 = f.iv0 - 1;
-            f.su = (int)floorf(fu * 1.05f + stride_margin) + 1;
-            f.sv = nv == 1 ? 1 : (int)floorf(fv * 1.05f + stride_margin) + 1;
-            // a stride wider than the footprint already puts every column
-            // (row) in its own phase
-            f.su = min(f.su, max(f.iu1 - f.iu0 + 1, 1));
-            f.sv = min(f.sv, max(f.iv1 - f.iv0 + 1, 1));
-        }
-        return f;
+        return vir::view_foot(g, proj + 12 * va, gb_lo, gb_hi, nu, nv, lane);
     };
 
     for (int va0 = a_begin; va0 < a_end; va0 += NS) {
