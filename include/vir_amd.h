@@ -149,6 +149,22 @@ int vir_siddon_bp_rec(const float *rec, int32_t nviews, int32_t nu, int32_t nv,
                       const float *sino, float *vol, uint32_t flags,
                       void *workspace, void *stream);
 
+/* One SIRT iteration (sys_mat/sirt_dd.py:51-72, tests/Test_SIRT/
+ * sirt_test.py:25-35) as two projector calls, the elementwise steps folded
+ * into the projectors' write-back:
+ *   vir_*_fp_res: r = (b - A x) / w_sino where w_sino != 0, else A x;
+ *   vir_*_bp_upd: x = x + gamma * u / w_img where w_img != 0 (else
+ *                 x + gamma * u), u = A^T r, x updated in place.
+ * Same bits as vir_siddon_fp_rec / vir_siddon_bp_rec followed by
+ * vir_sirt_residual / vir_sirt_update.  VIR_ACCUMULATE is rejected. */
+int vir_siddon_fp_rec_res(const float *rec, int64_t nrays, const vir_grid *grid,
+                          const float *x, const float *b, const float *w_sino,
+                          float *r, uint32_t flags, void *stream);
+int vir_siddon_bp_rec_upd(const float *rec, int32_t nviews, int32_t nu, int32_t nv,
+                          const float *proj, const vir_grid *grid, const float *r,
+                          const float *w_img, float gamma, float *x, uint32_t flags,
+                          void *workspace, void *stream);
+
 /* vir_siddon_bp_rec without the workspace argument: the same launches and
  * the same result, bit for bit, staging the view chunks' partial volumes
  * (or, with VIR_ACCUMULATE, the volume) in a fixed device array of
@@ -265,6 +281,15 @@ int vir_dd2d_fp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
 int vir_dd2d_bp(const vir_dd_geom *geom, const float *cosv, const float *sinv,
                 const float *sino, float *img, float out_scale, uint32_t flags,
                 void *stream);
+
+/* The fused SIRT steps of vir_siddon_fp_rec_res / vir_siddon_bp_rec_upd on
+ * the 2-D distance-driven projector (dd.py:675-803 with sirt_dd.py:51-72). */
+int vir_dd2d_fp_res(const vir_dd_geom *geom, const float *cosv, const float *sinv,
+                    const float *x, const float *b, const float *w_sino, float *r,
+                    float out_scale, uint32_t flags, void *stream);
+int vir_dd2d_bp_upd(const vir_dd_geom *geom, const float *cosv, const float *sinv,
+                    const float *r, const float *w_img, float gamma, float *x,
+                    float out_scale, uint32_t flags, void *stream);
 
 /* Cone-beam DD (dd.py:284-468, 815-888): sino (na, nu, nv); reference
  * scaling out_scale = d_pix^2/(du*dv).  VIR_COMPAT: the reference's sweep

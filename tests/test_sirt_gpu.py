@@ -119,3 +119,71 @@ def test_tv_denoise_reduces_total_variation():
         return np.abs(np.diff(a, axis=0)).sum() + np.abs(np.diff(a, axis=1)).sum()
     assert tv(den) < 0.5 * tv(noisy)
     assert np.linalg.norm(den - img) < np.linalg.norm(noisy - img)
+
+
+def _fused_case(kind):
+    import torch
+    from vir_amd.geometry import circular_projections, circular_rays
+    from vir_amd.serve import cone_geometry, fan_geometry
+    from vir_amd.sys_mat.dd import DDProjector
+    from vir_amd.sys_mat.siddon import SiddonProjector
+    from vir_amd.workloads import phantom_device
+    if kind == "fan":
+        g = fan_geometry(96, 60)
+        P = DDProjector(g["img_shape"], g["ang"], g["nu"], "fan", DSO=g["DSO"], DSD=g["DSD"],
+                        du=g["du"], detector="arc")
+    else:
+        # 24^3: the brick gather splits the views into chunks (partial
+        # volumes summed by the reduction kernel); 192^3: one chunk, the
+        # update is written by the gather itself
+        n, views = (24, 20) if kind == "cone_chunked" else (192, 4)
+        g = cone_geometry(n, views)
+        src, trg = circular_rays(g["dets_y"], g["views"], "cone", g["src_iso"], g["det_iso"],
+                                 g["dets_z"])
+        mats = circular_projections(g["dets_y"], g["views"], "cone", g["src_iso"], g["det_iso"],
+                                    g["dets_z"])
+        P = SiddonProjector(src, trg, g["img_shape"], 1.0, proj=mats)
+    b = P.fp(phantom_device(g["img_shape"], 8, 21))
+    torch.cuda.synchronize()
+    return P, b
+
+
+@pytest.mark.parametrize("kind", ["cone_chunked", "cone_direct", "fan"])
+def test_fused_sirt_steps_equal_separate_steps(kind):
+    """The residual written by the forward projection and the update
+    written by the backprojection give the iterates of the separate
+    elementwise steps bit for bit; graph replays give the eager iterates."""
+    import torch
+    from vir_amd import _lib
+    from vir_amd.sirt import SIRT
+    P, b = _fused_case(kind)
+    if kind == "cone_chunked":
+        assert int(_lib.lib().vir_siddon_bp_workspace(P.grid, P.layout[0])) > 0
+    elif kind == "cone_direct":
+        assert int(_lib.lib().vir_siddon_bp_workspace(P.grid, P.layout[0])) == 0
+    gamma = 0.9 if kind == "fan" else 1.0
+    ref = SIRT(P, b, gamma=gamma, fused=False)
+    B = SIRT(P, b, gamma=gamma, fused=True)
+    G = SIRT(P, b, gamma=gamma, fused=True, graph=True)
+    assert B.fused and not ref.fused and G.fused
+    for it in range(3):
+        ref.step()
+        B.step()
+        G.step()
+        torch.cuda.synchronize()
+        xr, xb, xg = (S.x.cpu().numpy() for S in (ref, B, G))
+        assert np.abs(xr).max() > 0
+        assert np.array_equal(xr.view(np.int32), xb.view(np.int32)), (kind, it)
+        assert np.array_equal(xr.view(np.int32), xg.view(np.int32)), (kind, it)
+    # the fused entry points refuse VIR_ACCUMULATE
+    import ctypes
+    from vir_amd.device import stream
+    if kind == "fan":
+        rc = _lib.lib().vir_dd2d_bp_upd(P.g, _lib.ptr(P.cos), _lib.ptr(P.sin), _lib.ptr(B.r),
+                                        _lib.ptr(B.W_img), ctypes.c_float(1.0), _lib.ptr(B.x),
+                                        P.fp_scale, _lib.VIR_ACCUMULATE, stream())
+    else:
+        rc = _lib.lib().vir_siddon_fp_rec_res(_lib.ptr(P.rec), P.nrays, P.grid, _lib.ptr(B.x),
+                                              _lib.ptr(b), _lib.ptr(B.W_sino), _lib.ptr(B.r),
+                                              _lib.VIR_ACCUMULATE, stream())
+    assert rc == _lib.VIR_EINVAL

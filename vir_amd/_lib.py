@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvir_amd.so")
 
 VIR_OK = 0
+VIR_EINVAL = -1
+VIR_EHIP = -2
 VIR_ACCUMULATE = 1
 VIR_SEQUENTIAL = 2
 VIR_UNGUARDED = 4
@@ -116,6 +118,8 @@ SIGNATURES = {
     "vir_siddon_ray_records": (ctypes.c_int, [P, P, I64, G, P, P]),
     "vir_siddon_fp_rec": (ctypes.c_int, [P, I64, G, P, P, U32, P, P]),
     "vir_siddon_bp_rec": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P, P]),
+    "vir_siddon_fp_rec_res": (ctypes.c_int, [P, I64, G, P, P, P, P, U32, P]),
+    "vir_siddon_bp_rec_upd": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, F32, P, U32, P, P]),
     "vir_siddon_bp_rec_noalloc": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P]),
     "vir_siddons_c_count": (ctypes.c_int, [P, P, I64, SCG, P, P, P, P, P]),
     "vir_siddons_c_fill": (ctypes.c_int, [P, P, I64, SCG, P, P, P, P, P, P, P]),
@@ -125,6 +129,8 @@ SIGNATURES = {
     "vir_csc_bp": (ctypes.c_int, [P, P, P, I64, P, P, U32, P]),
     "vir_dd2d_fp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_dd2d_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
+    "vir_dd2d_fp_res": (ctypes.c_int, [DG, P, P, P, P, P, P, F32, U32, P]),
+    "vir_dd2d_bp_upd": (ctypes.c_int, [DG, P, P, P, P, F32, P, F32, U32, P]),
     "vir_ddcone_fp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddcone_bp": (ctypes.c_int, [DG, P, P, P, P, F32, U32, P]),
     "vir_ddsq_fp": (ctypes.c_int, [QG, P, P, F32, U32, P]),
@@ -178,7 +184,7 @@ def check(rc, what=""):
     if rc != VIR_OK:
         msg = lib().vir_last_error()
         msg = msg.decode() if msg else ""
-        if rc == -1:
+        if rc == VIR_EINVAL:
             raise ValueError(f"{what}: {msg}")
         raise VirError(f"{what}: {msg} (rc={rc})")
 

@@ -123,7 +123,7 @@ __device__ __forceinline__ float overlap(float a0, float a1, float b0, float b1)
 template <int MODE>
 __global__ __launch_bounds__(256) void k_dd2d_fp(DDGeo G, const float* __restrict__ cs,
                                                 const float* __restrict__ sn, const float* __restrict__ img,
-                                                float* __restrict__ sino, int accumulate) {
+                                                float* __restrict__ sino, int accumulate, SirtEpi E) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)G.na * G.nu) return;
     const int ia = (int)(t / G.nu), iu = (int)(t % G.nu);
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_dd2d_fp(DDGeo G, const float* __restric
     }
     float r = (float)acc * G.out_scale;
     if (accumulate) r += sino[t];
-    sino[t] = r;
+    sirt_store(E, sino, t, r);
 }
 
 // Tiled forward: one workgroup per (view, 256 bins).  For every orthogonal
@@ -173,7 +173,7 @@ constexpr int kDDLines = 2048;  // lines whose scale and band are tabulated per 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __restrict__ cs,
                                                      const float* __restrict__ sn, const float* __restrict__ img,
-                                                     float* __restrict__ sino, int accumulate, int nut) {
+                                                     float* __restrict__ sino, int accumulate, int nut, SirtEpi E) {
     __shared__ float bnd[2][kDDKMax + 1];  // projected boundary kc + e of the two lines
     __shared__ float val[2][kDDKMax];      // pixel kc + e
     // per-line tables sized by the launch to max(nx, ny) lines (<= kDDLines):
@@ -276,14 +276,14 @@ __global__ __launch_bounds__(256) void k_dd2d_fp_tile(DDGeo G, const float* __re
     const int64_t t = (int64_t)ia * G.nu + iu;
     float r = (float)acc * G.out_scale;
     if (accumulate) r += sino[t];
-    sino[t] = r;
+    sirt_store(E, sino, t, r);
 }
 
 // One thread per pixel.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_dd2d_bp(DDGeo G, const float* __restrict__ cs,
                                                 const float* __restrict__ sn, const float* __restrict__ sino,
-                                                float* __restrict__ img, int accumulate) {
+                                                float* __restrict__ img, int accumulate, SirtEpi E) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)G.nx * G.ny) return;
     const int ix = (int)(t / G.ny), iy = (int)(t % G.ny);
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void k_dd2d_bp(DDGeo G, const float* __restric
     }
     float r = (float)acc * G.out_scale;
     if (accumulate) r += img[t];
-    img[t] = r;
+    sirt_store(E, img, t, r);
 }
 
 // ---------------------------------------------------------------- cone
@@ -510,10 +510,8 @@ static int dd_check(const vir_dd_geom* g, DDGeo& G, bool cone) {
 
 using namespace vir;
 
-extern "C" {
-
-int vir_dd2d_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* img, float* sino,
-                float out_scale, uint32_t flags, void* stream) {
+static int dd2d_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* img, float* sino,
+                   float out_scale, uint32_t flags, void* stream, const SirtEpi& E) {
     DDGeo G;
     int rc = dd_check(geom, G, false);
     if (rc) return rc;
@@ -530,21 +528,21 @@ int vir_dd2d_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, c
         dim3 tg((unsigned)((int64_t)G.na * nut));
         const size_t tables = (size_t)std::max(G.nx, G.ny) * (sizeof(float) + sizeof(short2));
         if (geom->mode == DD_PAR)
-            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_PAR>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut);
+            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_PAR>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut, E);
         else if (geom->mode == DD_FAN_FLAT)
-            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_FLAT>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut);
-        else hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_ARC>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut);
+            hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_FLAT>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut, E);
+        else hipLaunchKernelGGL(k_dd2d_fp_tile<DD_FAN_ARC>, tg, dim3(256), tables, st, G, cosv, sinv, img, sino, acc, nut, E);
         return check_hip(hipGetLastError(), "vir_dd2d_fp launch");
     }
-    if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_fp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
+    if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_fp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, E);
     else if (geom->mode == DD_FAN_FLAT)
-        hipLaunchKernelGGL(k_dd2d_fp<DD_FAN_FLAT>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
-    else hipLaunchKernelGGL(k_dd2d_fp<DD_FAN_ARC>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc);
+        hipLaunchKernelGGL(k_dd2d_fp<DD_FAN_FLAT>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, E);
+    else hipLaunchKernelGGL(k_dd2d_fp<DD_FAN_ARC>, grd, dim3(256), 0, st, G, cosv, sinv, img, sino, acc, E);
     return check_hip(hipGetLastError(), "vir_dd2d_fp launch");
 }
 
-int vir_dd2d_bp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* sino, float* img,
-                float out_scale, uint32_t flags, void* stream) {
+static int dd2d_bp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* sino, float* img,
+                   float out_scale, uint32_t flags, void* stream, const SirtEpi& E) {
     DDGeo G;
     int rc = dd_check(geom, G, false);
     if (rc) return rc;
@@ -555,11 +553,37 @@ int vir_dd2d_bp(const vir_dd_geom* geom, const float* cosv, const float* sinv, c
     dim3 grd((unsigned)((n + 255) / 256));
     int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
     hipStream_t st = as_stream(stream);
-    if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_bp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc);
+    if (geom->mode == DD_PAR) hipLaunchKernelGGL(k_dd2d_bp<DD_PAR>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc, E);
     else if (geom->mode == DD_FAN_FLAT)
-        hipLaunchKernelGGL(k_dd2d_bp<DD_FAN_FLAT>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc);
-    else hipLaunchKernelGGL(k_dd2d_bp<DD_FAN_ARC>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc);
+        hipLaunchKernelGGL(k_dd2d_bp<DD_FAN_FLAT>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc, E);
+    else hipLaunchKernelGGL(k_dd2d_bp<DD_FAN_ARC>, grd, dim3(256), 0, st, G, cosv, sinv, sino, img, acc, E);
     return check_hip(hipGetLastError(), "vir_dd2d_bp launch");
+}
+
+extern "C" {
+
+int vir_dd2d_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* img, float* sino,
+                float out_scale, uint32_t flags, void* stream) {
+    return dd2d_fp(geom, cosv, sinv, img, sino, out_scale, flags, stream, SirtEpi{});
+}
+
+int vir_dd2d_bp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* sino, float* img,
+                float out_scale, uint32_t flags, void* stream) {
+    return dd2d_bp(geom, cosv, sinv, sino, img, out_scale, flags, stream, SirtEpi{});
+}
+
+int vir_dd2d_fp_res(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* img, const float* b,
+                    const float* w_sino, float* r, float out_scale, uint32_t flags, void* stream) {
+    if (!b || !w_sino) return fail_arg("NULL pointer argument");
+    if (flags & VIR_ACCUMULATE) return fail_arg("VIR_ACCUMULATE does not apply to the fused residual");
+    return dd2d_fp(geom, cosv, sinv, img, r, out_scale, flags, stream, SirtEpi{b, w_sino, nullptr, 0.0f});
+}
+
+int vir_dd2d_bp_upd(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* r,
+                    const float* w_img, float gamma, float* x, float out_scale, uint32_t flags, void* stream) {
+    if (!w_img || !x) return fail_arg("NULL pointer argument");
+    if (flags & VIR_ACCUMULATE) return fail_arg("VIR_ACCUMULATE does not apply to the fused update");
+    return dd2d_bp(geom, cosv, sinv, r, x, out_scale, flags, stream, SirtEpi{nullptr, w_img, x, gamma});
 }
 
 int vir_ddcone_fp(const vir_dd_geom* geom, const float* cosv, const float* sinv, const float* vol, float* sino,

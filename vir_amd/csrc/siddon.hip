@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
                                                           const float* __restrict__ rec, int64_t nrays,
                                                           const float* __restrict__ vol,
                                                           float* __restrict__ sino, int accumulate,
-                                                          int32_t* __restrict__ err) {
+                                                          int32_t* __restrict__ err, SirtEpi E) {
     int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
     int64_t r = blk * kRayThreads + threadIdx.x;
     if (r >= nrays) return;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
     sum += (double)((pin ? pv : 0.0f) * pl);
     float res = (float)sum;
     if (accumulate) res += sino[r];
-    sino[r] = res;
+    sirt_store(E, sino, r, res);
     if (bad && err) atomicAdd(err, bad);
 }
 
@@ -281,7 +281,8 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                                                       int nv, const float* __restrict__ proj,
                                                       const float* __restrict__ sino,
                                                       float* __restrict__ out, int views_per_chunk,
-                                                      int nbx, int nby, int nbz, int64_t out_stride) {
+                                                      int nbx, int nby, int nbz, int64_t out_stride,
+                                                      SirtEpi E) {
     constexpr int kVox = SBX * SBY * SBZ;
     // NS views at a time per wave: lanes [slot*LPQ, (slot+1)*LPQ) work on view
     // va0 + slot and accumulate into accumulator copy `slot` of the
@@ -436,12 +437,12 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
         float sum = acc_w[li];
 #pragma unroll
         for (int s = 1; s < NS; ++s) sum += acc_w[s * kVox + li];
-        dst[gi] = sum;
+        sirt_store(E, dst, gi, sum);
     }
 }
 
 __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int64_t n, float* __restrict__ vol,
-                               int accumulate) {
+                               int accumulate, SirtEpi E) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = accumulate ? vol[i] : 0.0f;
@@ -453,7 +454,7 @@ __global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int6
         else c += (t - y) + s;
         s = y;
     }
-    vol[i] = s + c;
+    sirt_store(E, vol, i, s + c);
 }
 
 // Sub-brick shapes (one per wave, four waves per workgroup): 3D grids use
@@ -565,7 +566,8 @@ int vir_siddon_fill(const float* src, const float* trg, int64_t nrays, const vir
 // The forward projector from endpoints (rec == NULL) or from per-ray
 // records (src == trg == NULL).
 static int siddon_fp(const float* src, const float* trg, const float* rec, int64_t nrays, const vir_grid* grid,
-                     const float* vol, float* sino, uint32_t flags, int32_t* err_count, void* stream) {
+                     const float* vol, float* sino, uint32_t flags, int32_t* err_count, void* stream,
+                     const SirtEpi& E = SirtEpi{}) {
     GridK g;
     int rc = make_grid(grid, g);
     if (rc) return rc;
@@ -580,7 +582,7 @@ static int siddon_fp(const float* src, const float* trg, const float* rec, int64
     const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
 #define VIR_FP_LAUNCH(P2_, ISO_, UNG_)                                                                       \
     hipLaunchKernelGGL((k_siddon_fp<P2_, ISO_, UNG_>), dim3((unsigned)nb), dim3(kRayThreads), 0,             \
-                       as_stream(stream), g, src, trg, rec, nrays, vol, sino, acc, err_count)
+                       as_stream(stream), g, src, trg, rec, nrays, vol, sino, acc, err_count, E)
     if (ung) {
         if (p2 && iso) VIR_FP_LAUNCH(true, true, true);
         else if (p2) VIR_FP_LAUNCH(true, false, true);
@@ -636,7 +638,7 @@ int64_t vir_siddon_bp_workspace(const vir_grid* grid, int32_t nviews) {
 // per-ray records (src == trg == NULL).
 static int siddon_bp(const float* src, const float* trg, const float* rec, int32_t nviews, int32_t nu,
                      int32_t nv, const float* proj, const vir_grid* grid, const float* sino, float* vol,
-                     uint32_t flags, void* workspace, void* stream) {
+                     uint32_t flags, void* workspace, void* stream, const SirtEpi& E = SirtEpi{}) {
     GridK g;
     int rc = make_grid(grid, g);
     if (rc) return rc;
@@ -648,6 +650,8 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
     hipStream_t st = as_stream(stream);
     bool direct = (p.nchunks == 1) && !(flags & VIR_ACCUMULATE);
     float* out = vol;
+    // the fused SIRT step goes where the final volume is written
+    const SirtEpi Ek = direct ? E : SirtEpi{};
     if (!direct) {
         if (p.nchunks > 1 && !workspace) return fail_arg("workspace required (vir_siddon_bp_workspace)");
         if (p.nchunks == 1) {
@@ -675,7 +679,8 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
         if (ung) VIR_BP_LAUNCH1(SHAPE, true);                                                                    \
         else VIR_BP_LAUNCH1(SHAPE, false);                                                                       \
     } while (0)
-#define VIR_BP_ARGS g, src, trg, rec, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox
+#define VIR_BP_ARGS \
+    g, src, trg, rec, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox, Ek
 #define BP3_ARGS 16, 8, 8, 2, 2, 4, 2
 #define BP3_WX BpShape3::wx
 #define BP3_WY BpShape3::wy
@@ -698,7 +703,7 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
     if (!direct) {
         int64_t nbk = (nvox + 255) / 256;
         hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nbk), dim3(256), 0, st, out, p.nchunks, nvox, vol,
-                           (flags & VIR_ACCUMULATE) ? 1 : 0);
+                           (flags & VIR_ACCUMULATE) ? 1 : 0, E);
         rc = check_hip(hipGetLastError(), "vir_siddon_bp reduce launch");
     }
     return rc;
@@ -717,6 +722,25 @@ int vir_siddon_bp_rec(const float* rec, int32_t nviews, int32_t nu, int32_t nv, 
     if (!rec) return fail_arg("NULL pointer argument");
     if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
     return siddon_bp(nullptr, nullptr, rec, nviews, nu, nv, proj, grid, sino, vol, flags, workspace, stream);
+}
+
+int vir_siddon_fp_rec_res(const float* rec, int64_t nrays, const vir_grid* grid, const float* vol, const float* b,
+                          const float* w_sino, float* r, uint32_t flags, void* stream) {
+    if (nrays > 0 && (!rec || !b || !w_sino)) return fail_arg("NULL pointer argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    if (flags & VIR_ACCUMULATE) return fail_arg("VIR_ACCUMULATE does not apply to the fused residual");
+    return siddon_fp(nullptr, nullptr, rec, nrays, grid, vol, r, flags, nullptr, stream,
+                     SirtEpi{b, w_sino, nullptr, 0.0f});
+}
+
+int vir_siddon_bp_rec_upd(const float* rec, int32_t nviews, int32_t nu, int32_t nv, const float* proj,
+                          const vir_grid* grid, const float* r, const float* w_img, float gamma, float* x,
+                          uint32_t flags, void* workspace, void* stream) {
+    if (!rec || !w_img || !x) return fail_arg("NULL pointer argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    if (flags & VIR_ACCUMULATE) return fail_arg("VIR_ACCUMULATE does not apply to the fused update");
+    return siddon_bp(nullptr, nullptr, rec, nviews, nu, nv, proj, grid, r, x, flags, workspace, stream,
+                     SirtEpi{nullptr, w_img, x, gamma});
 }
 
 }  // extern "C"

@@ -7,17 +7,16 @@ __global__ void k_sirt_residual(const float* __restrict__ b, float* __restrict__
                                 int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float w = W[i];
-    if (w != 0.0f) r[i] = (b[i] - r[i]) / w;
+    const SirtEpi E{b, W, nullptr, 0.0f};
+    r[i] = sirt_residual(E, i, r[i]);
 }
 
 __global__ void k_sirt_update(float* __restrict__ x, const float* __restrict__ u, const float* __restrict__ W,
                               float gamma, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float w = W[i];
-    float q = w != 0.0f ? u[i] / w : u[i];
-    x[i] = x[i] + gamma * q;
+    const SirtEpi E{nullptr, W, x, gamma};
+    sirt_update(E, i, u[i]);
 }
 
 }  // namespace vir

@@ -25,6 +25,35 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nblocks) {
 
 int make_grid(const vir_grid* gp, GridK& g);
 
+// SIRT steps fused into a projector's write-back (sys_mat/sirt_dd.py:51-72),
+// with the arithmetic of the elementwise kernels vir_sirt_residual /
+// vir_sirt_update, so fused and separate iterations give the same bits.
+//   residual (b != NULL): out = (b - v) / w where w != 0, else v
+//   update   (x != NULL): x = x + gamma * (w != 0 ? v / w : v); out unused
+struct SirtEpi {
+    const float* b;
+    const float* w;
+    float* x;
+    float gamma;
+};
+
+__device__ __forceinline__ float sirt_residual(const SirtEpi& E, int64_t i, float v) {
+    const float w = E.w[i];
+    return w != 0.0f ? (E.b[i] - v) / w : v;
+}
+
+__device__ __forceinline__ void sirt_update(const SirtEpi& E, int64_t i, float u) {
+    const float w = E.w[i];
+    const float q = w != 0.0f ? u / w : u;
+    E.x[i] = E.x[i] + E.gamma * q;
+}
+
+// Stores v at out[i], or the fused SIRT step of v.
+__device__ __forceinline__ void sirt_store(const SirtEpi& E, float* out, int64_t i, float v) {
+    if (E.x) sirt_update(E, i, v);
+    else out[i] = E.b ? sirt_residual(E, i, v) : v;
+}
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace vir

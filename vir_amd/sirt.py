@@ -8,10 +8,14 @@ tests/Test_SIRT/2d_sirt.py:18-35, processing/SIRT/sirt.py:14-44):
     u = A^T r / W_img          where W_img  != 0   (else u = A^T r)
     x = x + gamma * u
 
-with W_sino = A 1 and W_img = A^T 1 (sirt_test.py:157-158).  The
-weighted-residual and update steps are one elementwise kernel each
-(vir_sirt_residual, vir_sirt_update); one iteration is four launches on one
-stream and can be captured once into a HIP graph and replayed.
+with W_sino = A 1 and W_img = A^T 1 (sirt_test.py:157-158).  Projectors
+that offer it (fused_sirt: the Siddon brick gather from per-ray records, the
+2-D distance-driven pair) write the weighted residual from the forward
+projection's and the update from the backprojection's write-back, so an
+iteration is two launches; otherwise the two steps are elementwise kernels
+(vir_sirt_residual, vir_sirt_update) and an iteration is four launches.
+Both give the same bits.  An iteration can be captured once into a HIP
+graph and replayed.
 
 Any projector with fp(x, out=) / bp(y, out=) on CUDA tensors works:
 SiddonProjector, DDProjector, StoredMatrix (below).
@@ -86,9 +90,11 @@ class SIRT:
     place on self.x; run(n) runs n; graph=True captures one iteration into a
     HIP graph on first use and replays it."""
 
-    def __init__(self, P, b, x0=None, W_sino=None, W_img=None, gamma=1.0, graph=False):
+    def __init__(self, P, b, x0=None, W_sino=None, W_img=None, gamma=1.0, graph=False, fused=True):
+        """fused: use the projector's fused steps when it has them."""
         require_gpu()
         self.P = P
+        self.fused = bool(fused) and bool(getattr(P, "fused_sirt", False))
         self.img_shape, self.sino_shape = _shapes(P)
         self.b = as_device_f32(b).reshape(self.sino_shape).contiguous()
         dev = self.b.device
@@ -108,6 +114,10 @@ class SIRT:
         self.iterations = 0
 
     def _iteration(self):
+        if self.fused:
+            self.P.fp_residual(self.x, self.b, self.W_sino, self.r)
+            self.P.bp_update(self.r, self.W_img, self.gamma, self.x)
+            return
         L = _lib.lib()
         st = stream()
         self.P.fp(self.x, out=self.r)

@@ -123,6 +123,29 @@ class DDProjector:
         return out
 
 
+    # -- SIRT steps fused into the projections (2-D geometries) ------------
+    @property
+    def fused_sirt(self):
+        return self.geom != "cone"
+
+    def fp_residual(self, img, b, w_sino, out):
+        """out = (b - A img) / w_sino where w_sino != 0, else A img (the SIRT
+        residual written by the forward projection; same bits as fp()
+        followed by the separate residual step)."""
+        _lib.check(_lib.lib().vir_dd2d_fp_res(self.g, _lib.ptr(self.cos), _lib.ptr(self.sin), _lib.ptr(img),
+                                              _lib.ptr(b), _lib.ptr(w_sino), _lib.ptr(out), self.fp_scale, 0,
+                                              stream()), "vir_dd2d_fp_res")
+        return out
+
+    def bp_update(self, r, w_img, gamma, x):
+        """x += gamma * (A^T r) / w_img (where w_img != 0, else A^T r), in
+        place: the SIRT update written by the backprojection."""
+        _lib.check(_lib.lib().vir_dd2d_bp_upd(self.g, _lib.ptr(self.cos), _lib.ptr(self.sin), _lib.ptr(r),
+                                              _lib.ptr(w_img), float(gamma), _lib.ptr(x), self.fp_scale, 0,
+                                              stream()), "vir_dd2d_bp_upd")
+        return x
+
+
 def _wrap(x, torch_in):
     return x if torch_in else x.cpu().numpy()
 

@@ -265,6 +265,32 @@ class SiddonProjector:
                                          _lib.ptr(sino), _lib.ptr(out), fl, stream()), "vir_csc_bp")
         return out
 
+    # -- SIRT steps fused into the projections ----------------------------
+    @property
+    def fused_sirt(self):
+        """True when fp_residual / bp_update are available (the brick-gather
+        backprojector from per-ray records)."""
+        return self.rec is not None and self.proj is not None
+
+    def fp_residual(self, vol, b, w_sino, out):
+        """out = (b - A vol) / w_sino where w_sino != 0, else A vol: the SIRT
+        residual written by the forward projection (same bits as fp()
+        followed by the separate residual step)."""
+        _lib.check(_lib.lib().vir_siddon_fp_rec_res(_lib.ptr(self.rec), self.nrays, self.grid, _lib.ptr(vol),
+                                                    _lib.ptr(b), _lib.ptr(w_sino), _lib.ptr(out), self._ung,
+                                                    stream()), "vir_siddon_fp_rec_res")
+        return out
+
+    def bp_update(self, r, w_img, gamma, x):
+        """x += gamma * (A^T r) / w_img (where w_img != 0, else A^T r), in
+        place: the SIRT update written by the backprojection."""
+        nviews, nu, nv = self.layout
+        _lib.check(_lib.lib().vir_siddon_bp_rec_upd(_lib.ptr(self.rec), nviews, nu, nv, _lib.ptr(self.proj),
+                                                    self.grid, _lib.ptr(r), _lib.ptr(w_img), float(gamma),
+                                                    _lib.ptr(x), self._ung, _lib.ptr(self._ws), stream()),
+                   "vir_siddon_bp_rec_upd")
+        return x
+
     def stored_csr(self):
         """(off int64, ravel_idx int32, len f32, flags) device tensors."""
         off, flags, rav, ix, iy, iz, ln = _trace(self.src, self.trg, self.grid, self.nrays)
