@@ -36,6 +36,10 @@ def parse():
                     help="only run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the fan-beam SIRT, differentiable-projector and serving sections")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving section (profiling runs)")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 error checks (profiling runs)")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the Poisson trace replayed by the serving section")
     ap.add_argument("--plain-requests", type=int, default=6,
@@ -270,7 +274,7 @@ def events(n):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
 
-def sirt_fan_section():
+def sirt_fan_section(fp64=True):
     """512^2 ellipse phantom, 720 views over 2*pi, 888 equiangular bins
     (DSO 1000, DSD 1500 pixel units), Poisson noise I0 = 1e5 on the
     log-transformed sinogram, 50 graph-captured SIRT iterations with the
@@ -303,7 +307,7 @@ def sirt_fan_section():
     e1.record()
     torch.cuda.synchronize()
     ms_it = e0.elapsed_time(e1) / 49
-    fp64 = fan_fp64_check(P, g, b, x_true)
+    fp64 = fan_fp64_check(P, g, b, x_true) if fp64 else None
     torch.cuda.empty_cache()
     n_views, n_d, n_v = 720, g["nu"], 512 * 512
     b_fp = n_views * (4.0 * n_v + 4.0 * n_d)
@@ -378,7 +382,7 @@ def fan_fp64_check(P, g, b, x_true, iters=2):
     return out
 
 
-def autograd_section():
+def autograd_section(fp64=True):
     """Batch of 8 random-ellipsoid volumes 256x256x64, 3-D parallel beam,
     180 views over [0, pi), 384 x 64 detector at 1.0: 100 gradient steps on
     0.5*||A x - b||^2 + 1e-3*TV(x), A the separable-footprint projector
@@ -400,13 +404,16 @@ def autograd_section():
     rhs = float((xr.double() * P.bp(yr).double()).sum())
     adj = abs(lhs - rhs) / abs(lhs)
     # FP64: one volume of the batch on 4 of the 180 views
-    sub = ang[::45]
-    P1 = SFProjector(shape, sub, 384, "par3d", nv=64, du=1.0, dv=1.0)
-    y1 = torch.rand(P1.sino_shape, device="cuda", generator=gen)
-    f64, b64 = sf_par3d_fp64(truth[0], P1.cos.cpu().numpy(), P1.sin.cpu().numpy(), 384, 64, y=y1)
-    fp64 = {"views": [int(round(float(a) * 180 / np.pi)) for a in sub],
-            "fp": _err(P1.fp(truth[0]), f64), "bp": _err(P1.bp(y1), b64)}
-    del P1, y1, f64, b64
+    if fp64:
+        sub = ang[::45]
+        P1 = SFProjector(shape, sub, 384, "par3d", nv=64, du=1.0, dv=1.0)
+        y1 = torch.rand(P1.sino_shape, device="cuda", generator=gen)
+        f64, b64 = sf_par3d_fp64(truth[0], P1.cos.cpu().numpy(), P1.sin.cpu().numpy(), 384, 64, y=y1)
+        fp64 = {"views": [int(round(float(a) * 180 / np.pi)) for a in sub],
+                "fp": _err(P1.fp(truth[0]), f64), "bp": _err(P1.bp(y1), b64)}
+        del P1, y1, f64, b64
+    else:
+        fp64 = None
     v = xr / xr.norm()
     for _ in range(8):             # power iteration for ||A||^2
         w = P.bp(P.fp(v))
@@ -588,25 +595,27 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_kernel),
                      "alg_bytes": dom_bytes, "l2_hit_rate": l2_hit_rate(dom_kernel)},
     }
-    log("[bench] FP64 check on 4 views")
     del sino, back
-    out["fp64_error"] = fp64_siddon_check(vol)
-    torch.cuda.empty_cache()
-    log(f"[bench] {out['fp64_error']}")
+    if not args.no_fp64:
+        log("[bench] FP64 check on 4 views")
+        out["fp64_error"] = fp64_siddon_check(vol)
+        torch.cuda.empty_cache()
+        log(f"[bench] {out['fp64_error']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(src, trg, grid, args.cpu_seconds)
     if not args.no_extras:
         del P, vol, src, trg
         torch.cuda.empty_cache()
         log("[bench] fan-beam SIRT section")
-        out["sirt_fan"] = sirt_fan_section()
+        out["sirt_fan"] = sirt_fan_section(fp64=not args.no_fp64)
         log(f"[bench] {out['sirt_fan']}")
         log("[bench] differentiable projector section")
-        out["autograd"] = autograd_section()
+        out["autograd"] = autograd_section(fp64=not args.no_fp64)
         log(f"[bench] {out['autograd']}")
         torch.cuda.empty_cache()
-        log("[bench] serving section")
-        out["serving"] = serving_section(args.serve_requests, args.plain_requests)
+        if not args.no_serve:
+            log("[bench] serving section")
+            out["serving"] = serving_section(args.serve_requests, args.plain_requests)
     print(json.dumps(out), flush=True)
 
 
