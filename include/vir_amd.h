@@ -165,6 +165,37 @@ int vir_siddon_bp_rec_upd(const float *rec, int32_t nviews, int32_t nu, int32_t 
                           const float *w_img, float gamma, float *x, uint32_t flags,
                           void *workspace, void *stream);
 
+/* Batched forms of vir_siddon_fp_rec, vir_siddon_bp_rec and the fused SIRT
+ * steps above, for nbatch >= 1 volumes that share the rays: volume j at
+ * vol + j*nX*nY*nZ, its sinogram at sino + j*nrays (nrays = nviews*nu*nv
+ * for the BP); for the fused steps b, r and x are batched likewise and the
+ * weights w_sino, w_img are shared.  They replace the same reference
+ * interfaces (projection/proj.py:56-150, sys_mat/sirt_dd.py:51-72) called
+ * once per volume.  Each ray is traced once for a group of up to four
+ * volumes (FP) or two sinograms (BP), so the trace is paid once per group;
+ * every item's result equals the single-volume call on it, bit for bit.
+ * err_count receives the rays' out-of-grid entries once.  The BP workspace
+ * is vir_siddon_bp_batch_workspace() bytes, or min(nbatch, 2)*nX*nY*nZ*4
+ * with VIR_ACCUMULATE on a single-chunk plan. */
+int vir_siddon_fp_rec_batch(const float *rec, int64_t nrays, const vir_grid *grid,
+                            int32_t nbatch, const float *vol, float *sino,
+                            uint32_t flags, int32_t *err_count, void *stream);
+int64_t vir_siddon_bp_batch_workspace(const vir_grid *grid, int32_t nviews,
+                                      int32_t nbatch);
+int vir_siddon_bp_rec_batch(const float *rec, int32_t nviews, int32_t nu, int32_t nv,
+                            const float *proj, const vir_grid *grid, int32_t nbatch,
+                            const float *sino, float *vol, uint32_t flags,
+                            void *workspace, void *stream);
+int vir_siddon_fp_rec_res_batch(const float *rec, int64_t nrays, const vir_grid *grid,
+                                int32_t nbatch, const float *x, const float *b,
+                                const float *w_sino, float *r, uint32_t flags,
+                                void *stream);
+int vir_siddon_bp_rec_upd_batch(const float *rec, int32_t nviews, int32_t nu, int32_t nv,
+                                const float *proj, const vir_grid *grid, int32_t nbatch,
+                                const float *r, const float *w_img, float gamma,
+                                float *x, uint32_t flags, void *workspace,
+                                void *stream);
+
 /* vir_siddon_bp_rec without the workspace argument: the same launches and
  * the same result, bit for bit, staging the view chunks' partial volumes
  * (or, with VIR_ACCUMULATE, the volume) in a fixed device array of

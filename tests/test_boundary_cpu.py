@@ -70,6 +70,19 @@ def test_invalid_arguments_rejected():
     assert lib.vir_siddon_bp_rec(ctypes.c_void_p(64), 1, 1 << 15, 1 << 15, ctypes.c_void_p(64), g,
                                  ctypes.c_void_p(64), ctypes.c_void_p(64), 0, None, None) == -1
     assert b"nu*nv" in lib.vir_last_error()
+    # batches: at least one item, records required, fused forms refuse VIR_ACCUMULATE
+    P64 = ctypes.c_void_p(64)
+    assert lib.vir_siddon_fp_rec_batch(P64, 1, g, 0, P64, P64, 0, None, None) == -1
+    assert b"nbatch" in lib.vir_last_error()
+    assert lib.vir_siddon_fp_rec_batch(None, 1, g, 2, P64, P64, 0, None, None) == -1
+    assert lib.vir_siddon_fp_rec_batch(None, 0, g, 2, None, None, 0, None, None) == 0  # nothing to do
+    assert lib.vir_siddon_bp_batch_workspace(g, 1, 0) == -1
+    assert lib.vir_siddon_bp_batch_workspace(g, 1, 3) == 2 * lib.vir_siddon_bp_workspace(g, 1)
+    assert lib.vir_siddon_bp_rec_batch(P64, 1, 1, 1, P64, g, 0, P64, P64, 0, None, None) == -1
+    assert lib.vir_siddon_bp_rec_batch(None, 1, 1, 1, P64, g, 2, P64, P64, 0, None, None) == -1
+    assert lib.vir_siddon_fp_rec_res_batch(P64, 1, g, 2, P64, P64, P64, P64, _lib.VIR_ACCUMULATE, None) == -1
+    assert lib.vir_siddon_bp_rec_upd_batch(P64, 1, 1, 1, P64, g, 2, P64, P64, 1.0, P64, _lib.VIR_ACCUMULATE,
+                                           None, None) == -1
 
 
 @pytest.mark.parametrize("name,kw", [

@@ -120,6 +120,11 @@ SIGNATURES = {
     "vir_siddon_bp_rec": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P, P]),
     "vir_siddon_fp_rec_res": (ctypes.c_int, [P, I64, G, P, P, P, P, U32, P]),
     "vir_siddon_bp_rec_upd": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, F32, P, U32, P, P]),
+    "vir_siddon_fp_rec_batch": (ctypes.c_int, [P, I64, G, I32, P, P, U32, P, P]),
+    "vir_siddon_bp_batch_workspace": (I64, [G, I32, I32]),
+    "vir_siddon_bp_rec_batch": (ctypes.c_int, [P, I32, I32, I32, P, G, I32, P, P, U32, P, P]),
+    "vir_siddon_fp_rec_res_batch": (ctypes.c_int, [P, I64, G, I32, P, P, P, P, U32, P]),
+    "vir_siddon_bp_rec_upd_batch": (ctypes.c_int, [P, I32, I32, I32, P, G, I32, P, P, F32, P, U32, P, P]),
     "vir_siddon_bp_rec_noalloc": (ctypes.c_int, [P, I32, I32, I32, P, G, P, P, U32, P]),
     "vir_siddons_c_count": (ctypes.c_int, [P, P, I64, SCG, P, P, P, P, P]),
     "vir_siddons_c_fill": (ctypes.c_int, [P, P, I64, SCG, P, P, P, P, P, P, P]),

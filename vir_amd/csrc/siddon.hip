@@ -119,18 +119,20 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_ray_records(GridK g, con
 
 // ------------------------------------------------------------ forward
 
-template <bool P2, bool ISO, bool UNG>
-__global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_siddon_fp(GridK g, const float* __restrict__ src,
-                                                          const float* __restrict__ trg,
-                                                          const float* __restrict__ rec, int64_t nrays,
-                                                          const float* __restrict__ vol,
-                                                          float* __restrict__ sino, int accumulate,
-                                                          int32_t* __restrict__ err, SirtEpi E) {
+// NB volumes that share the rays (volume j at vol + j*vstride, its
+// projection at sino + j*nrays): one trace per ray serves all of them, and
+// each volume's sums are formed exactly as for NB = 1.
+template <bool P2, bool ISO, bool UNG, int NB = 1>
+__global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 7 : 6))) void k_siddon_fp(
+    GridK g, const float* __restrict__ src, const float* __restrict__ trg, const float* __restrict__ rec,
+    int64_t nrays, const float* __restrict__ vol, float* __restrict__ sino, int accumulate,
+    int32_t* __restrict__ err, SirtEpi E, int64_t vstride) {
     int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
     int64_t r = blk * kRayThreads + threadIdx.x;
     if (r >= nrays) return;
     Ray R;
-    if (rec) ray_load_record(rec + (size_t)kRayRec * r, R);  // the same setup, precomputed
+    // batched calls always come with records
+    if (NB > 1 || rec) ray_load_record(rec + (size_t)kRayRec * r, R);  // the same setup, precomputed
     else ray_setup<UNG>(g, src + 3 * r, trg + 3 * r, R);
     // float64 accumulation of the float32 products (the accumulator of
     // projection/proj_numba.py:17), one-deep software pipeline: the gather
@@ -138,9 +140,15 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
     // consumed before the gather for j+1 overwrites it.  Every pair issues a
     // gather (of voxel 0 for dropped and out-of-grid pairs, weighted 0), so
     // the body has no branch around the load.
-    double sum = 0.0;
+    double sum[NB];
+    float pv[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        sum[j] = 0.0;
+        pv[j] = 0.0f;
+    }
     int bad = 0;
-    float pv = 0.0f, pl = 0.0f;
+    float pl = 0.0f;
     bool pin = false;
     if (R.hit) {
         Walker W;
@@ -156,17 +164,25 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(7))
                     // iy, n2 <= 2^22 and iy n2 < n1 n2: a 24-bit (full-rate)
                     // multiply is exact for in-grid entries
                     unsigned vi = (unsigned)ix * nyz + __umul24((unsigned)iy, (unsigned)g.n[2]) + (unsigned)iz;
-                    sum += (double)((pin ? pv : 0.0f) * pl);
-                    pv = vol[use ? vi : 0u];
+                    const unsigned gi = use ? vi : 0u;
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        sum[j] += (double)((pin ? pv[j] : 0.0f) * pl);
+                        pv[j] = vol[j * vstride + gi];
+                    }
                     pin = use;
                     pl = use ? len : 0.0f;
                 });
         });
     }
-    sum += (double)((pin ? pv : 0.0f) * pl);
-    float res = (float)sum;
-    if (accumulate) res += sino[r];
-    sirt_store(E, sino, r, res);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        sum[j] += (double)((pin ? pv[j] : 0.0f) * pl);
+        float res = (float)sum[j];
+        float* sj = sino + j * nrays;
+        if (accumulate) res += sj[r];
+        sirt_store(sirt_item(E, j * nrays), sj, r, res);
+    }
     if (bad && err) atomicAdd(err, bad);
 }
 
@@ -274,26 +290,33 @@ __device__ __forceinline__ ViewFoot view_foot(const GridK& g, const float* __res
 // footprint apart, so the lanes of a wave never update the same voxel in the
 // same colour phase, and consecutive phases of one wave are ordered by the
 // in-order LDS pipeline -- no atomics and no workgroup barriers.
-template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, int NS, bool P2, bool ISO, bool UNG>
-__global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WPE))) void k_siddon_bp_wave(GridK g, const float* __restrict__ src,
-                                                      const float* __restrict__ trg,
-                                                      const float* __restrict__ rec, int nviews, int nu,
-                                                      int nv, const float* __restrict__ proj,
-                                                      const float* __restrict__ sino,
-                                                      float* __restrict__ out, int views_per_chunk,
-                                                      int nbx, int nby, int nbz, int64_t out_stride,
-                                                      SirtEpi E) {
+//
+// NB sinograms that share the rays (sinogram j at sino + j*sstride, volume j
+// at out + j*nvox): one trace per candidate ray serves all of them, each
+// into its own accumulator set, and every volume receives the additions of
+// an NB = 1 run in the same order (a ray whose weight is zero for one
+// sinogram adds a zero product to it, which leaves every sum's bits as they
+// were).
+// (NB accumulator sets take NB times the LDS, so NB > 1 runs at two waves
+// per SIMD.)
+template <int SBX, int SBY, int SBZ, int WX, int WY, int WPE, int NS, bool P2, bool ISO, bool UNG, int NB = 1>
+__global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB == 1 ? WPE : 2))) void k_siddon_bp_wave(
+    GridK g, const float* __restrict__ src, const float* __restrict__ trg, const float* __restrict__ rec,
+    int nviews, int nu, int nv, const float* __restrict__ proj, const float* __restrict__ sino,
+    float* __restrict__ out, int views_per_chunk, int nbx, int nby, int nbz, int64_t out_stride, SirtEpi E,
+    int64_t sstride, int64_t nvox) {
     constexpr int kVox = SBX * SBY * SBZ;
     // NS views at a time per wave: lanes [slot*LPQ, (slot+1)*LPQ) work on view
     // va0 + slot and accumulate into accumulator copy `slot` of the
     // sub-brick (copies summed in slot order at the end), so rays of
     // different views never meet in one accumulator
     constexpr int LPQ = 64 / NS;
-    __shared__ float acc_all[WX * WY * NS * kVox];
+    __shared__ float acc_all[WX * WY * NB * NS * kVox];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int slot = lane / LPQ, lq = lane % LPQ;
-    float* acc_w = acc_all + wave * NS * kVox;
+    // accumulator copy `slot` of batch item j at acc_w + (j NS + slot) kVox
+    float* acc_w = acc_all + wave * NB * NS * kVox;
     float* acc = acc_w + slot * kVox;
 
     // nbx/nby/nbz count workgroup regions (WX*SBX x WY*SBY x SBZ)
@@ -305,7 +328,7 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
     const int b1[3] = {min(b0[0] + SBX, g.n[0]), min(b0[1] + SBY, g.n[1]), min(b0[2] + SBZ, g.n[2])};
     const bool live = b0[0] < g.n[0] && b0[1] < g.n[1];
 
-    for (int i = lane; i < NS * kVox; i += 64) acc_w[i] = 0.0f;
+    for (int i = lane; i < NB * NS * kVox; i += 64) acc_w[i] = 0.0f;
 
     // guard box: the sub-brick grown by kGuard voxels.  An entry whose
     // computed index falls in the sub-brick has its true midpoint within
@@ -373,11 +396,18 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     const unsigned ro = __umul24((unsigned)iu, (unsigned)nv) + (unsigned)iv;
                     // zero-weight rays (they miss the object) skip the
                     // endpoint loads altogether
-                    float wgt = sino_v[ro];
-                    if (wgt == 0.0f) continue;
+                    float wgt[NB];
+                    bool live_w = false;
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) {
+                        wgt[j] = sino_v[j * sstride + ro];
+                        live_w |= wgt[j] != 0.0f;
+                    }
+                    if (!live_w) continue;
                     Ray R;
-                    if (rec) {
-                        // precomputed setup and exact slices (uniform branch)
+                    if (NB > 1 || rec) {
+                        // precomputed setup and exact slices (uniform branch;
+                        // batched calls always come with records)
                         ray_load_record(rec_v + (size_t)kRayRec * ro, R);
                     } else {
                         const float* sr = src_v + 3 * ro;
@@ -403,7 +433,7 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                     }
                     if (miss || alo > ahi) continue;
                     // exact plane slices only matter near the ray's ends
-                    if (!rec && !window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
+                    if (NB == 1 && !rec && !window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
                     walk_dispatch<UNG>(g, R, [&](auto fast) {
                         constexpr bool F = decltype(fast)::value;
                         Walker W;
@@ -413,7 +443,9 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
                             if ((unsigned)lx < (unsigned)(b1[0] - b0[0]) &&
                                 (unsigned)ly < (unsigned)(b1[1] - b0[1]) &&
                                 (unsigned)lz < (unsigned)(b1[2] - b0[2])) {
-                                acc[(lx * SBY + ly) * SBZ + lz] += wgt * len;
+                                const int li = (lx * SBY + ly) * SBZ + lz;
+#pragma unroll
+                                for (int j = 0; j < NB; ++j) acc[j * NS * kVox + li] += wgt[j] * len;
                             }
                         });
                     });
@@ -434,21 +466,26 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(WP
         int lz = i % ez, ly = (i / ez) % ey, lx = i / (ez * ey);
         int64_t gi = ((int64_t)(b0[0] + lx) * g.n[1] + (b0[1] + ly)) * g.n[2] + (b0[2] + lz);
         const int li = (lx * SBY + ly) * SBZ + lz;
-        float sum = acc_w[li];
 #pragma unroll
-        for (int s = 1; s < NS; ++s) sum += acc_w[s * kVox + li];
-        sirt_store(E, dst, gi, sum);
+        for (int j = 0; j < NB; ++j) {
+            const float* aj = acc_w + j * NS * kVox;
+            float sum = aj[li];
+#pragma unroll
+            for (int s = 1; s < NS; ++s) sum += aj[s * kVox + li];
+            sirt_store(sirt_item(E, j * nvox), dst + j * nvox, gi, sum);
+        }
     }
 }
 
-__global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int64_t n, float* __restrict__ vol,
-                               int accumulate, SirtEpi E) {
+// part: nchunks partial volumes of n voxels, chunk k at part + k*cstride.
+__global__ void k_sum_partials(const float* __restrict__ part, int nchunks, int64_t n, int64_t cstride,
+                               float* __restrict__ vol, int accumulate, SirtEpi E) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = accumulate ? vol[i] : 0.0f;
     float c = 0.0f;
     for (int k = 0; k < nchunks; ++k) {
-        float t = part[(int64_t)k * n + i];
+        float t = part[(int64_t)k * cstride + i];
         float y = s + t;
         if (fabsf(s) >= fabsf(t)) c += (s - y) + t;
         else c += (t - y) + s;
@@ -565,24 +602,19 @@ int vir_siddon_fill(const float* src, const float* trg, int64_t nrays, const vir
 
 // The forward projector from endpoints (rec == NULL) or from per-ray
 // records (src == trg == NULL).
-static int siddon_fp(const float* src, const float* trg, const float* rec, int64_t nrays, const vir_grid* grid,
-                     const float* vol, float* sino, uint32_t flags, int32_t* err_count, void* stream,
-                     const SirtEpi& E = SirtEpi{}) {
-    GridK g;
-    int rc = make_grid(grid, g);
-    if (rc) return rc;
-    if (nrays < 0) return fail_arg("nrays < 0");
-    if (nrays > 0 && !(rec || (src && trg))) return fail_arg("src/trg is NULL");
-    if ((int64_t)g.n[0] * g.n[1] * g.n[2] >= ((int64_t)1 << 32))
-        return fail_arg("volume must have fewer than 2^32 voxels");
-    if (nrays == 0) return VIR_OK;
-    if (!vol || !sino) return fail_arg("vol/sino is NULL");
-    int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
-    int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
-    const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
+// Largest number of volumes one FP launch traces together; larger batches
+// run as groups of at most this many.
+constexpr int kFpBatch = 4;
+
+extern "C++" {
+template <int NB>
+static void siddon_fp_launch(const GridK& g, const float* src, const float* trg, const float* rec, int64_t nrays,
+                             const float* vol, float* sino, int acc, int32_t* err_count, hipStream_t st,
+                             const SirtEpi& E, int64_t nvox, bool p2, bool iso, bool ung) {
+    const int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
 #define VIR_FP_LAUNCH(P2_, ISO_, UNG_)                                                                       \
-    hipLaunchKernelGGL((k_siddon_fp<P2_, ISO_, UNG_>), dim3((unsigned)nb), dim3(kRayThreads), 0,             \
-                       as_stream(stream), g, src, trg, rec, nrays, vol, sino, acc, err_count, E)
+    hipLaunchKernelGGL((k_siddon_fp<P2_, ISO_, UNG_, NB>), dim3((unsigned)nb), dim3(kRayThreads), 0, st, g,  \
+                       src, trg, rec, nrays, vol, sino, acc, err_count, E, nvox)
     if (ung) {
         if (p2 && iso) VIR_FP_LAUNCH(true, true, true);
         else if (p2) VIR_FP_LAUNCH(true, false, true);
@@ -595,7 +627,44 @@ static int siddon_fp(const float* src, const float* trg, const float* rec, int64
         else VIR_FP_LAUNCH(false, false, false);
     }
 #undef VIR_FP_LAUNCH
-    return check_hip(hipGetLastError(), "vir_siddon_fp launch");
+}
+}  // extern "C++"
+
+// nbatch volumes (volume j at vol + j*nvox, projection at sino + j*nrays;
+// records only when nbatch > 1), traced in groups of kFpBatch, 2 and 1.
+static int siddon_fp(const float* src, const float* trg, const float* rec, int64_t nrays, const vir_grid* grid,
+                     const float* vol, float* sino, uint32_t flags, int32_t* err_count, void* stream,
+                     const SirtEpi& E = SirtEpi{}, int32_t nbatch = 1) {
+    GridK g;
+    int rc = make_grid(grid, g);
+    if (rc) return rc;
+    if (nrays < 0) return fail_arg("nrays < 0");
+    if (nbatch < 1) return fail_arg("nbatch must be >= 1");
+    if (nrays > 0 && !(rec || (src && trg))) return fail_arg("src/trg is NULL");
+    if (nbatch > 1 && nrays > 0 && !rec) return fail_arg("batched projection needs ray records");
+    const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    if (nvox >= ((int64_t)1 << 32)) return fail_arg("volume must have fewer than 2^32 voxels");
+    if (nrays == 0) return VIR_OK;
+    if (!vol || !sino) return fail_arg("vol/sino is NULL");
+    int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
+    const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
+    hipStream_t st = as_stream(stream);
+    for (int32_t j0 = 0; j0 < nbatch;) {
+        const int32_t left = nbatch - j0;
+        const int nb = left >= kFpBatch ? kFpBatch : (left >= 2 ? 2 : 1);
+        const float* vj = vol + j0 * nvox;
+        float* sj = sino + j0 * nrays;
+        const SirtEpi Ej = sirt_item(E, j0 * nrays);
+        if (nb == kFpBatch) siddon_fp_launch<kFpBatch>(g, src, trg, rec, nrays, vj, sj, acc, err_count, st, Ej, nvox, p2, iso, ung);
+        else if (nb == 2) siddon_fp_launch<2>(g, src, trg, rec, nrays, vj, sj, acc, err_count, st, Ej, nvox, p2, iso, ung);
+        else siddon_fp_launch<1>(g, src, trg, rec, nrays, vj, sj, acc, err_count, st, Ej, nvox, p2, iso, ung);
+        rc = check_hip(hipGetLastError(), "vir_siddon_fp launch");
+        if (rc) return rc;
+        // out-of-grid entries are a property of the rays: count them once
+        err_count = nullptr;
+        j0 += nb;
+    }
+    return VIR_OK;
 }
 
 int vir_siddon_fp(const float* src, const float* trg, int64_t nrays, const vir_grid* grid, const float* vol,
@@ -634,61 +703,81 @@ int64_t vir_siddon_bp_workspace(const vir_grid* grid, int32_t nviews) {
     return (int64_t)p.nchunks * nvox * (int64_t)sizeof(float);
 }
 
+// Largest number of sinograms one BP launch backprojects together (each
+// adds a set of sub-brick accumulators to the workgroup's LDS); larger
+// batches run as groups.
+constexpr int kBpBatch = 2;
+
 // The brick-gather backprojector from endpoints (rec == NULL) or from
-// per-ray records (src == trg == NULL).
+// per-ray records (src == trg == NULL), for nbatch sinograms (sinogram j at
+// sino + j*nviews*nu*nv, volume j at vol + j*nvox; records only when
+// nbatch > 1), backprojected in groups of kBpBatch and 1.  The workspace
+// holds one group's staging.
 static int siddon_bp(const float* src, const float* trg, const float* rec, int32_t nviews, int32_t nu,
                      int32_t nv, const float* proj, const vir_grid* grid, const float* sino, float* vol,
-                     uint32_t flags, void* workspace, void* stream, const SirtEpi& E = SirtEpi{}) {
+                     uint32_t flags, void* workspace, void* stream, const SirtEpi& E = SirtEpi{},
+                     int32_t nbatch = 1) {
     GridK g;
     int rc = make_grid(grid, g);
     if (rc) return rc;
     if (nviews < 1 || nu < 1 || nv < 1) return fail_arg("nviews/nu/nv must be >= 1");
     if ((int64_t)nu * nv >= ((int64_t)1 << 30)) return fail_arg("nu*nv must be < 2^30");
+    if (nbatch < 1) return fail_arg("nbatch must be >= 1");
     if (!(rec || (src && trg)) || !proj || !sino || !vol) return fail_arg("NULL pointer argument");
+    if (nbatch > 1 && !rec) return fail_arg("batched backprojection needs ray records");
     BpPlan p = plan_bp(g, nviews);
-    int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t nvox = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t nrays = (int64_t)nviews * nu * nv;
     hipStream_t st = as_stream(stream);
-    bool direct = (p.nchunks == 1) && !(flags & VIR_ACCUMULATE);
-    float* out = vol;
-    // the fused SIRT step goes where the final volume is written
-    const SirtEpi Ek = direct ? E : SirtEpi{};
+    const bool direct = (p.nchunks == 1) && !(flags & VIR_ACCUMULATE);
     if (!direct) {
         if (p.nchunks > 1 && !workspace) return fail_arg("workspace required (vir_siddon_bp_workspace)");
-        if (p.nchunks == 1) {
-            // accumulate with a single chunk: stage through the workspace if
-            // provided, otherwise this path needs one
-            if (!workspace) return fail_arg("workspace required for VIR_ACCUMULATE");
-        }
-        out = reinterpret_cast<float*>(workspace);
+        // accumulate with a single chunk: stage through the workspace
+        if (p.nchunks == 1 && !workspace) return fail_arg("workspace required for VIR_ACCUMULATE");
     }
-    int nb = p.nbx * p.nby * p.nbz;
-    dim3 grd((unsigned)nb, (unsigned)p.nchunks);
+    const int nb = p.nbx * p.nby * p.nbz;
+    const dim3 grd((unsigned)nb, (unsigned)p.nchunks);
     const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
-#define VIR_BP_LAUNCH1(SHAPE, UNG_)                                                                            \
-    do {                                                                                                         \
-        dim3 blk(64 * (SHAPE##_WX) * (SHAPE##_WY));                                                              \
-        if (p2 && iso) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, true, UNG_>), grd, blk, 0, st,    \
-                                          VIR_BP_ARGS);                                                          \
-        else if (p2) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, false, UNG_>), grd, blk, 0, st,     \
-                                        VIR_BP_ARGS);                                                            \
-        else hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, false, false, UNG_>), grd, blk, 0, st,            \
-                                VIR_BP_ARGS);                                                                    \
+    for (int32_t j0 = 0; j0 < nbatch;) {
+        const int NBg = nbatch - j0 >= kBpBatch ? kBpBatch : 1;
+        const float* sino_g = sino + j0 * nrays;
+        float* vol_g = vol + j0 * nvox;
+        const SirtEpi Eg = sirt_item(E, j0 * nvox);
+        // staging: chunk k of item j at out + (k NBg + j) nvox
+        float* out = direct ? vol_g : reinterpret_cast<float*>(workspace);
+        const int64_t out_stride = (int64_t)NBg * nvox;
+        // the fused SIRT step goes where the final volume is written
+        const SirtEpi Ek = direct ? Eg : SirtEpi{};
+#define VIR_BP_LAUNCH1(SHAPE, UNG_, NB_)                                                                      \
+    do {                                                                                                        \
+        dim3 blk(64 * (SHAPE##_WX) * (SHAPE##_WY));                                                             \
+        if (p2 && iso) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, true, UNG_, NB_>), grd, blk, 0,  \
+                                          st, VIR_BP_ARGS);                                                     \
+        else if (p2) hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, true, false, UNG_, NB_>), grd, blk, 0,   \
+                                        st, VIR_BP_ARGS);                                                       \
+        else hipLaunchKernelGGL((k_siddon_bp_wave<SHAPE##_ARGS, false, false, UNG_, NB_>), grd, blk, 0, st,      \
+                                VIR_BP_ARGS);                                                                   \
     } while (0)
-#define VIR_BP_LAUNCH(SHAPE)                                                                                   \
-    do {                                                                                                         \
-        if (ung) VIR_BP_LAUNCH1(SHAPE, true);                                                                    \
-        else VIR_BP_LAUNCH1(SHAPE, false);                                                                       \
+#define VIR_BP_LAUNCH(SHAPE)                                                                                  \
+    do {                                                                                                        \
+        if (NBg == kBpBatch) {                                                                                  \
+            if (ung) VIR_BP_LAUNCH1(SHAPE, true, kBpBatch);                                                     \
+            else VIR_BP_LAUNCH1(SHAPE, false, kBpBatch);                                                        \
+        } else {                                                                                                \
+            if (ung) VIR_BP_LAUNCH1(SHAPE, true, 1);                                                            \
+            else VIR_BP_LAUNCH1(SHAPE, false, 1);                                                               \
+        }                                                                                                       \
     } while (0)
 #define VIR_BP_ARGS \
-    g, src, trg, rec, nviews, nu, nv, proj, sino, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, nvox, Ek
+    g, src, trg, rec, nviews, nu, nv, proj, sino_g, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, out_stride, Ek, nrays, nvox
 #define BP3_ARGS 16, 8, 8, 2, 2, 4, 2
 #define BP3_WX BpShape3::wx
 #define BP3_WY BpShape3::wy
 #define BP2_ARGS 32, 16, 1, 2, 2, 4, 4
 #define BP2_WX 2
 #define BP2_WY 2
-    if (p.kind == 0) VIR_BP_LAUNCH(BP3);
-    else VIR_BP_LAUNCH(BP2);
+        if (p.kind == 0) VIR_BP_LAUNCH(BP3);
+        else VIR_BP_LAUNCH(BP2);
 #undef BP3_ARGS
 #undef BP3_WX
 #undef BP3_WY
@@ -698,15 +787,21 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
 #undef VIR_BP_ARGS
 #undef VIR_BP_LAUNCH
 #undef VIR_BP_LAUNCH1
-    rc = check_hip(hipGetLastError(), "vir_siddon_bp launch");
-    if (rc) return rc;
-    if (!direct) {
-        int64_t nbk = (nvox + 255) / 256;
-        hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nbk), dim3(256), 0, st, out, p.nchunks, nvox, vol,
-                           (flags & VIR_ACCUMULATE) ? 1 : 0, E);
-        rc = check_hip(hipGetLastError(), "vir_siddon_bp reduce launch");
+        rc = check_hip(hipGetLastError(), "vir_siddon_bp launch");
+        if (rc) return rc;
+        if (!direct) {
+            const int64_t nbk = (nvox + 255) / 256;
+            for (int j = 0; j < NBg; ++j) {
+                hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nbk), dim3(256), 0, st, out + j * nvox, p.nchunks,
+                                   nvox, out_stride, vol_g + j * nvox, (flags & VIR_ACCUMULATE) ? 1 : 0,
+                                   sirt_item(Eg, j * nvox));
+                rc = check_hip(hipGetLastError(), "vir_siddon_bp reduce launch");
+                if (rc) return rc;
+            }
+        }
+        j0 += NBg;
     }
-    return rc;
+    return VIR_OK;
 }
 
 int vir_siddon_bp(const float* src, const float* trg, int32_t nviews, int32_t nu, int32_t nv,
@@ -741,6 +836,50 @@ int vir_siddon_bp_rec_upd(const float* rec, int32_t nviews, int32_t nu, int32_t 
     if (flags & VIR_ACCUMULATE) return fail_arg("VIR_ACCUMULATE does not apply to the fused update");
     return siddon_bp(nullptr, nullptr, rec, nviews, nu, nv, proj, grid, r, x, flags, workspace, stream,
                      SirtEpi{nullptr, w_img, x, gamma});
+}
+
+// ---------------------------------------------- batches sharing the rays
+
+int vir_siddon_fp_rec_batch(const float* rec, int64_t nrays, const vir_grid* grid, int32_t nbatch,
+                            const float* vol, float* sino, uint32_t flags, int32_t* err_count, void* stream) {
+    if (nrays > 0 && !rec) return fail_arg("rec is NULL");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    return siddon_fp(nullptr, nullptr, rec, nrays, grid, vol, sino, flags, err_count, stream, SirtEpi{}, nbatch);
+}
+
+int64_t vir_siddon_bp_batch_workspace(const vir_grid* grid, int32_t nviews, int32_t nbatch) {
+    if (nbatch < 1) return -1;
+    const int64_t one = vir_siddon_bp_workspace(grid, nviews);
+    return one < 0 ? one : one * (nbatch < kBpBatch ? nbatch : kBpBatch);
+}
+
+int vir_siddon_bp_rec_batch(const float* rec, int32_t nviews, int32_t nu, int32_t nv, const float* proj,
+                            const vir_grid* grid, int32_t nbatch, const float* sino, float* vol, uint32_t flags,
+                            void* workspace, void* stream) {
+    if (!rec) return fail_arg("NULL pointer argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    return siddon_bp(nullptr, nullptr, rec, nviews, nu, nv, proj, grid, sino, vol, flags, workspace, stream,
+                     SirtEpi{}, nbatch);
+}
+
+int vir_siddon_fp_rec_res_batch(const float* rec, int64_t nrays, const vir_grid* grid, int32_t nbatch,
+                                const float* x, const float* b, const float* w_sino, float* r, uint32_t flags,
+                                void* stream) {
+    if (nrays > 0 && (!rec || !b || !w_sino)) return fail_arg("NULL pointer argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    if (flags & VIR_ACCUMULATE) return fail_arg("VIR_ACCUMULATE does not apply to the fused residual");
+    return siddon_fp(nullptr, nullptr, rec, nrays, grid, x, r, flags, nullptr, stream,
+                     SirtEpi{b, w_sino, nullptr, 0.0f}, nbatch);
+}
+
+int vir_siddon_bp_rec_upd_batch(const float* rec, int32_t nviews, int32_t nu, int32_t nv, const float* proj,
+                                const vir_grid* grid, int32_t nbatch, const float* r, const float* w_img,
+                                float gamma, float* x, uint32_t flags, void* workspace, void* stream) {
+    if (!rec || !w_img || !x) return fail_arg("NULL pointer argument");
+    if (reinterpret_cast<uintptr_t>(rec) % 16) return fail_arg("rec must be 16-byte aligned");
+    if (flags & VIR_ACCUMULATE) return fail_arg("VIR_ACCUMULATE does not apply to the fused update");
+    return siddon_bp(nullptr, nullptr, rec, nviews, nu, nv, proj, grid, r, x, flags, workspace, stream,
+                     SirtEpi{nullptr, w_img, x, gamma}, nbatch);
 }
 
 }  // extern "C"

@@ -48,6 +48,12 @@ __device__ __forceinline__ void sirt_update(const SirtEpi& E, int64_t i, float u
     E.x[i] = E.x[i] + E.gamma * q;
 }
 
+// The epilogue of a batch item whose per-item arrays (b, x) start `off`
+// elements further on; the weights are shared by the batch.
+__host__ __device__ inline SirtEpi sirt_item(const SirtEpi& E, int64_t off) {
+    return SirtEpi{E.b ? E.b + off : nullptr, E.w, E.x ? E.x + off : nullptr, E.gamma};
+}
+
 // Stores v at out[i], or the fused SIRT step of v.
 __device__ __forceinline__ void sirt_store(const SirtEpi& E, float* out, int64_t i, float v) {
     if (E.x) sirt_update(E, i, v);

@@ -291,6 +291,92 @@ class SiddonProjector:
                    "vir_siddon_bp_rec_upd")
         return x
 
+    # -- batches of volumes on these rays ---------------------------------
+    def _batch(self, t, per_item, what):
+        t = as_device_f32(t)
+        if t.numel() == 0 or t.numel() % per_item:
+            raise ValueError(f"{what} size is not a whole number of items")
+        return t, t.numel() // per_item
+
+    def _batch_ws(self, nbatch, accumulate):
+        """Staging for a batched backprojection (one group of sinograms)."""
+        nviews = self.layout[0]
+        need = int(_lib.lib().vir_siddon_bp_batch_workspace(self.grid, nviews, nbatch))
+        if accumulate:
+            need = max(need, min(nbatch, 2) * self.nvox * 4)
+        if self._ws.numel() * 4 >= need:
+            return self._ws
+        if getattr(self, "_wsb", None) is None or self._wsb.numel() * 4 < need:
+            self._wsb = torch.empty(need // 4 + 1, dtype=torch.float32, device=self.src.device)
+        return self._wsb
+
+    def _require_batch_path(self):
+        if self.rec is None or self.proj is None:
+            raise ValueError("batched projection needs the brick-gather layout with ray records")
+
+    def fp_batch(self, vols, out=None, accumulate=False, check=False):
+        """sino[j] = A vols[j] for a batch of volumes (B, *grid) on these rays.
+        The rays are traced once per group of four volumes; every item equals
+        fp(vols[j]) bit for bit."""
+        self._require_batch_path()
+        vols, nb = self._batch(vols, self.nvox, "volume")
+        out = out_tensor(out, (nb,) + tuple(self.ray_shape), vols.device)
+        if check:
+            self._err.zero_()
+        fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._ung
+        err = _lib.ptr(self._err) if check else None
+        _lib.check(_lib.lib().vir_siddon_fp_rec_batch(_lib.ptr(self.rec), self.nrays, self.grid, nb,
+                                                      _lib.ptr(vols), _lib.ptr(out), fl, err, stream()),
+                   "vir_siddon_fp_rec_batch")
+        if check and int(self._err.item()):
+            raise ValueError("invalid entry in coordinates array")
+        return out
+
+    def bp_batch(self, sinos, out=None, accumulate=False):
+        """vols[j] = A^T sinos[j] for a batch of sinograms (B, *ray_shape).
+        Each candidate ray is traced once per group of two sinograms; every
+        item equals bp(sinos[j]) bit for bit."""
+        self._require_batch_path()
+        sinos, nb = self._batch(sinos, self.nrays, "sinogram")
+        out = out_tensor(out, (nb,) + tuple(self.nPixels), sinos.device)
+        fl = (_lib.VIR_ACCUMULATE if accumulate else 0) | self._ung
+        nviews, nu, nv = self.layout
+        ws = self._batch_ws(nb, accumulate)
+        _lib.check(_lib.lib().vir_siddon_bp_rec_batch(_lib.ptr(self.rec), nviews, nu, nv, _lib.ptr(self.proj),
+                                                      self.grid, nb, _lib.ptr(sinos), _lib.ptr(out), fl,
+                                                      _lib.ptr(ws), stream()), "vir_siddon_bp_rec_batch")
+        return out
+
+    def fp_residual_batch(self, vols, b, w_sino, out):
+        """fp_residual for a batch: out[j] = (b[j] - A vols[j]) / w_sino."""
+        self._require_batch_path()
+        vols, nb = self._batch(vols, self.nvox, "volume")
+        b, w_sino = as_device_f32(b), as_device_f32(w_sino)
+        out = out_tensor(out, (nb,) + tuple(self.ray_shape), vols.device)
+        if b.numel() != nb * self.nrays or w_sino.numel() != self.nrays:
+            raise ValueError("b must hold one sinogram per volume, w_sino one sinogram")
+        _lib.check(_lib.lib().vir_siddon_fp_rec_res_batch(_lib.ptr(self.rec), self.nrays, self.grid, nb,
+                                                          _lib.ptr(vols), _lib.ptr(b), _lib.ptr(w_sino),
+                                                          _lib.ptr(out), self._ung, stream()),
+                   "vir_siddon_fp_rec_res_batch")
+        return out
+
+    def bp_update_batch(self, r, w_img, gamma, x):
+        """bp_update for a batch: x[j] += gamma * (A^T r[j]) / w_img, in place."""
+        self._require_batch_path()
+        r, nb = self._batch(r, self.nrays, "sinogram")
+        w_img = as_device_f32(w_img)
+        x = out_tensor(x, (nb,) + tuple(self.nPixels), r.device, "x")
+        if w_img.numel() != self.nvox:
+            raise ValueError("w_img must hold one volume")
+        nviews, nu, nv = self.layout
+        ws = self._batch_ws(nb, False)
+        _lib.check(_lib.lib().vir_siddon_bp_rec_upd_batch(_lib.ptr(self.rec), nviews, nu, nv,
+                                                          _lib.ptr(self.proj), self.grid, nb, _lib.ptr(r),
+                                                          _lib.ptr(w_img), float(gamma), _lib.ptr(x), self._ung,
+                                                          _lib.ptr(ws), stream()), "vir_siddon_bp_rec_upd_batch")
+        return x
+
     def stored_csr(self):
         """(off int64, ravel_idx int32, len f32, flags) device tensors."""
         off, flags, rav, ix, iy, iz, ln = _trace(self.src, self.trg, self.grid, self.nrays)
