@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC calibration)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the fan-beam SIRT, differentiable-projector and serving sections")
+                    help="skip the batched-volume, fan-beam SIRT, differentiable-projector and serving sections")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving section (profiling runs)")
     ap.add_argument("--no-fp64", action="store_true",
@@ -446,6 +446,41 @@ def autograd_section(fp64=True):
             "alg_bytes_fp": b_fp, "lr": lr}
 
 
+def batched_section(P, vol, fp_ms, bp_ms, steps=3):
+    """FP and BP of B volumes on the same rays (one trace per group): time
+    per call and per volume against the single-volume calls, and item 0
+    checked bit for bit against the single call."""
+    import torch
+    res = {}
+    single_f = P.fp(vol)
+    single_b = P.bp(single_f)
+    for nb in (2, 4):
+        vols = torch.stack([vol * (1.0 + 0.25 * j) for j in range(nb)])
+        sinos = P.fp_batch(vols)
+        backs = P.bp_batch(sinos)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(sinos[0].view(torch.int32), single_f.view(torch.int32)) and
+                    torch.equal(backs[0].view(torch.int32), single_b.view(torch.int32)))
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        tf, tb = [], []
+        for _ in range(steps):
+            e[0].record()
+            P.fp_batch(vols, out=sinos)
+            e[1].record()
+            P.bp_batch(sinos, out=backs)
+            e[2].record()
+            torch.cuda.synchronize()
+            tf.append(e[0].elapsed_time(e[1]))
+            tb.append(e[1].elapsed_time(e[2]))
+        f, b = float(np.mean(tf)), float(np.mean(tb))
+        res[f"batch{nb}"] = {"fp_ms": f, "bp_ms": b, "fp_ms_per_volume": f / nb, "bp_ms_per_volume": b / nb,
+                             "fp_speedup_per_volume": fp_ms * nb / f, "bp_speedup_per_volume": bp_ms * nb / b,
+                             "item0_equals_single_call": same}
+        del vols, sinos, backs
+        torch.cuda.empty_cache()
+    return res
+
+
 def serving_section(n_serve, n_plain):
     """The serving trace (Poisson arrivals at 200/s, 60 % fan-beam jobs of
     128..1024^2 and 180..720 views, 40 % cone-beam jobs of 64..256^3 and
@@ -464,7 +499,7 @@ def serving_section(n_serve, n_plain):
         warm, keep_results=False)
     be = SirtBackend(n_streams=8, cache_bytes=32 << 30)
     sample = trace[:n_serve]
-    srv = Server(be, 200 << 30, max_active=16, group=True)
+    srv = Server(be, 200 << 30, max_active=16, group=True, batch=True)
     res = srv.run(sample, keep_results=False,
                   progress=lambda r: log(f"[serve] request {r.rid} done in {r.latency:.2f}s")
                   if r.rid % 16 == 0 else None)
@@ -473,7 +508,7 @@ def serving_section(n_serve, n_plain):
                 "max_concurrent": srv.max_concurrent, "max_in_use_GB": srv.max_in_use / 1e9,
                 "projector_cache": {"hits": be.cache_hits, "misses": be.cache_misses,
                                     "limit_GB": 32},
-                "grouped_admissions": srv.grouped,
+                "grouped_admissions": srv.grouped, "batched_admissions": srv.batched,
                 "trace": "poisson_trace(10000, 200/s, seed 1005)[:%d]" % n_serve})
     del be, srv, res
     torch.cuda.empty_cache()
@@ -596,6 +631,10 @@ def main():
                      "alg_bytes": dom_bytes, "l2_hit_rate": l2_hit_rate(dom_kernel)},
     }
     del sino, back
+    if not args.no_extras:
+        log("[bench] batched volumes on the same rays")
+        out["batched"] = batched_section(P, vol, fp_ms, bp_ms)
+        log(f"[bench] {out['batched']}")
     if not args.no_fp64:
         log("[bench] FP64 check on 4 views")
         out["fp64_error"] = fp64_siddon_check(vol)

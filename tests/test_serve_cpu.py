@@ -116,6 +116,56 @@ def test_grouped_admission_pulls_same_geometry_forward():
     assert [e[1] for e in be.log if e[0] == "start"] == [0, 1, 2, 3, 4]
 
 
+class BatchingBackend(KeyedBackend):
+    """Cone requests of one (n, views, iters) batch, up to 3 at once; rid 9
+    makes its whole batch fail to start."""
+    max_batch = 3
+
+    @staticmethod
+    def batch_key(req):
+        return None if req.kind != "cone" else (req.n, req.views, req.iters)
+
+    def start_batch(self, reqs, slots):
+        assert 1 < len(reqs) <= self.max_batch and len(set(slots)) == len(slots)
+        assert len({self.batch_key(r) for r in reqs}) == 1
+        if any(r.rid == 9 for r in reqs):
+            raise RuntimeError("batch setup failed")
+        self.log.append(("batch", tuple(r.rid for r in reqs)))
+        for r, s in zip(reqs, slots):
+            self.log.append(("start", r.rid, s))
+        return [FakeHandle(r, self.log) for r in reqs]
+
+
+def test_batched_admission():
+    reqs = [Request(0, "cone", 2, 2), Request(1, "fan", 2, 2), Request(2, "cone", 2, 2),
+            Request(3, "cone", 2, 2, iters=5), Request(4, "cone", 2, 2), Request(5, "cone", 2, 2),
+            Request(6, "cone", 4, 2)]
+    be = BatchingBackend()
+    srv = Server(be, hbm_budget=100_000, max_active=8, batch=True)
+    res = srv.run(reqs)
+    assert all(r.ok and r.x[0] == r.rid for r in res)
+    batches = [e[1] for e in be.log if e[0] == "batch"]
+    # 0 takes the next two of its key (up to max_batch = 3); 5 is left for
+    # a batch of one; other keys and the fan request run alone
+    assert batches == [(0, 2, 4)]
+    assert srv.batched == 2
+    rel = sorted(e[1] for e in be.log if e[0] == "release")
+    assert rel == list(range(7)) and srv.in_use == 0
+    # the budget caps a batch: with room for two, request 4 waits
+    be = BatchingBackend()
+    srv = Server(be, hbm_budget=4_000, max_active=8, batch=True)
+    assert all(r.ok for r in srv.run(reqs[:5]))
+    assert [e[1] for e in be.log if e[0] == "batch"][0] == (0, 2)
+    assert srv.max_in_use <= 4_000
+    # a failing batch fails all of its members and frees their slots
+    reqs = [Request(8, "cone", 2, 2), Request(9, "cone", 2, 2), Request(10, "fan", 2, 2)]
+    be = BatchingBackend()
+    srv = Server(be, hbm_budget=100_000, max_active=2, batch=True)
+    res = srv.run(reqs)
+    assert [r.ok for r in res] == [False, False, True]
+    assert "batch setup failed" in res[0].error and srv.in_use == 0
+
+
 def test_poisson_trace_mix():
     tr = poisson_trace(4000, 200.0, seed=5)
     kinds = np.array([r.kind == "fan" for r in tr])

@@ -181,3 +181,23 @@ def test_cached_and_grouped_geometries_give_the_alone_results():
     assert all(r.ok for r in res0) and be0.cache_hits == 0 and not be0._cache
     for r, q in zip(res0, reqs[:3]):
         assert np.array_equal(r.x, run_alone(q)), q
+
+
+def test_batched_cone_jobs_give_the_alone_results():
+    """Cone requests of one geometry, iteration count and step admitted
+    together run as one batched SIRT (the rays traced once per group in
+    every projection); each result is bitwise that of the request run
+    alone.  Batches of 3 (FP group of 3 as 2 + 1, BP groups of 2 + 1) and 2,
+    with a fan request and a cone request of another key in between."""
+    from vir_amd.serve import Request, Server, SirtBackend, run_alone
+    reqs = [Request(0, "cone", 24, 20, iters=3, seed=11), Request(1, "fan", 32, 20, iters=2, seed=12),
+            Request(2, "cone", 24, 20, iters=3, seed=13), Request(3, "cone", 24, 20, iters=4, seed=14),
+            Request(4, "cone", 24, 20, iters=3, seed=15), Request(5, "cone", 24, 20, iters=4, seed=16)]
+    be = SirtBackend(n_streams=2, cache_bytes=1 << 30)
+    srv = Server(be, hbm_budget=8 << 30, max_active=8, batch=True)
+    res = srv.run(reqs)
+    assert all(r.ok for r in res), [r.error for r in res if not r.ok]
+    assert srv.batched == 3   # (0, 2, 4) and (3, 5)
+    for r, q in zip(res, reqs):
+        assert np.array_equal(r.x, run_alone(q)), q
+    assert srv.in_use == 0 and all(G.users == 0 for G in be._cache.values())

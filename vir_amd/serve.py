@@ -26,7 +26,11 @@ the backend keeps the projector and its normalisation sums W_sino, W_img in
 an LRU cache bounded in bytes, and the server (group=True) admits waiting
 requests of the geometry it has just admitted right after it.  Jobs that
 share a projector run on one stream (the projector's backprojection
-workspace is per projector).
+workspace is per projector).  With batch=True the server admits up to
+backend.max_batch waiting requests of one batch key (cone beam: geometry,
+iterations and step) together, and the backend runs them as one batched
+SIRT whose projections trace the rays once for the whole group; each
+request's result is still bit-identical to its run alone.
 
 Fan-beam requests use the distance-driven projector with an equiangular
 detector (sys_mat/dd.py, the reference's fan geometry on an arc); cone-beam
@@ -108,12 +112,19 @@ class Server:
     handle.release().  Slots index the backend's streams."""
 
     def __init__(self, backend, hbm_budget, max_active=16, clock=time.perf_counter,
-                 idle_sleep=2e-4, group=False):
+                 idle_sleep=2e-4, group=False, batch=False):
         """group: after admitting a request, admit the waiting requests
-        with the same backend.geometry_key next (out of arrival order)."""
+        with the same backend.geometry_key next (out of arrival order).
+        batch: admit waiting requests with the admitted one's
+        backend.batch_key (not None) together, up to backend.max_batch, as
+        one backend.start_batch call (each keeps its own slot and budget
+        share)."""
         self.backend = backend
         self.group_key = getattr(backend, "geometry_key", None) if group else None
+        self.batch_key = getattr(backend, "batch_key", None) if batch else None
+        self.max_batch = int(getattr(backend, "max_batch", 1))
         self.grouped = 0
+        self.batched = 0
         self.budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.clock = clock
@@ -164,15 +175,36 @@ class Server:
                 waiting.popleft()
                 slot = free_slots.pop()
                 t_adm = self.clock() - t0
+                members = [(req, slot, est)]
+                key = self.batch_key(req) if self.batch_key is not None else None
+                if key is not None:
+                    need = self.in_use + est
+                    for w in list(waiting):
+                        if len(members) >= self.max_batch or not free_slots:
+                            break
+                        if self.batch_key(w) != key:
+                            continue
+                        e_w = int(self.backend.estimate(w))
+                        if need + e_w > self.budget:
+                            break
+                        waiting.remove(w)
+                        members.append((w, free_slots.pop(), e_w))
+                        need += e_w
                 try:
-                    h = self.backend.start(req, slot)
+                    if len(members) == 1:
+                        handles = [self.backend.start(req, slot)]
+                    else:
+                        handles = self.backend.start_batch([m[0] for m in members], [m[1] for m in members])
                 except Exception as e:  # one failing request does not stop the others
-                    free_slots.append(slot)
-                    results[req.rid] = Result(req.rid, False, error=f"{type(e).__name__}: {e}")
+                    for m in members:
+                        free_slots.append(m[1])
+                        results[m[0].rid] = Result(m[0].rid, False, error=f"{type(e).__name__}: {e}")
                     progressed = True
                     continue
-                active[slot] = (req, h, est, t_adm)
-                self.in_use += est
+                for (m_req, m_slot, m_est), h in zip(members, handles):
+                    active[m_slot] = (m_req, h, m_est, t_adm)
+                    self.in_use += m_est
+                self.batched += len(members) - 1
                 self.max_in_use = max(self.max_in_use, self.in_use)
                 self.max_concurrent = max(self.max_concurrent, len(active))
                 progressed = True
@@ -315,12 +347,89 @@ class _Job:
             self.geometry = None
 
 
+class _BatchJob:
+    """Requests of one geometry, iteration count and step run as one batched
+    SIRT (cone beam: the Siddon projector's batched fused steps)."""
+
+    def __init__(self, reqs, stream, graph=True, geometry=None):
+        import torch
+        from .sirt import SIRT
+        from .workloads import phantom_device
+        self.reqs = list(reqs)
+        self.stream = stream
+        G = geometry if geometry is not None else _Geometry(self.reqs[0], stream)
+        G.users += 1
+        self.geometry = G
+        with torch.cuda.stream(stream):
+            P, g = G.P, G.g
+            b = torch.empty((len(self.reqs),) + tuple(P.ray_shape), dtype=torch.float32, device="cuda")
+            for j, req in enumerate(self.reqs):
+                truth = phantom_device(g["img_shape"], 20, req.seed, stream=stream)
+                P.fp(truth, out=b[j])   # each measurement as its job alone makes it
+                del truth
+            S = SIRT(P, b, gamma=self.reqs[0].gamma, W_sino=G.W_sino, W_img=G.W_img, nbatch=len(self.reqs))
+            del b
+            self.S = S
+            self.host = torch.empty(S.x.shape, dtype=torch.float32, pin_memory=True)
+            iters = self.reqs[0].iters
+            if graph:
+                gr = torch.cuda.CUDAGraph()
+                gr.capture_begin()
+                try:
+                    S._iteration()
+                finally:
+                    gr.capture_end()
+                self.graph = gr
+                for _ in range(iters):
+                    gr.replay()
+            else:
+                self.graph = None
+                for _ in range(iters):
+                    S._iteration()
+            self.host.copy_(S.x, non_blocking=True)
+            self.done = torch.cuda.Event()
+            self.done.record(stream)
+        self.open = len(self.reqs)
+
+    def member(self, j):
+        return _Member(self, j)
+
+    def release_one(self):
+        self.open -= 1
+        if self.open == 0:
+            self.graph = None
+            self.S = None
+            self.geometry.users -= 1
+            self.geometry = None
+
+
+class _Member:
+    """The handle of one request of a batched job."""
+
+    def __init__(self, job, j):
+        self.job, self.j = job, j
+
+    def poll(self):
+        return self.job.done.query()
+
+    def result(self):
+        self.job.done.synchronize()
+        return self.job.host[self.j].numpy().copy()
+
+    def release(self):
+        self.job.release_one()
+
+
 class SirtBackend:
     """Runs requests as device-resident, graph-captured SIRT jobs.
 
     cache_bytes: device bytes of projectors (with W_sino, W_img) kept for
     later requests of the same geometry (least recently used evicted first,
-    never while a job uses it); 0 disables the cache."""
+    never while a job uses it); 0 disables the cache.  Cone-beam requests of
+    one geometry, iteration count and step can run batched (start_batch, up
+    to max_batch at once)."""
+
+    max_batch = 4
 
     def __init__(self, n_streams=8, graph=True, cache_bytes=32 << 30):
         import torch
@@ -336,6 +445,14 @@ class SirtBackend:
     @staticmethod
     def geometry_key(req):
         return (req.kind, req.n, req.views)
+
+    @staticmethod
+    def batch_key(req):
+        """Requests with one batch key run as one batched job (cone beam
+        only: the Siddon projector has the batched steps)."""
+        if req.kind != "cone":
+            return None
+        return (req.kind, req.n, req.views, req.iters, req.gamma)
 
     @staticmethod
     def estimate(req):
@@ -368,7 +485,7 @@ class SirtBackend:
                 del self._cache[k]
                 total -= e.nbytes
 
-    def start(self, req, slot):
+    def _geometry(self, req, slot):
         key = self.geometry_key(req)
         G = self._cache.get(key)
         if G is not None:
@@ -383,9 +500,23 @@ class SirtBackend:
             G = _Geometry(req, stream)
             if 0 < G.nbytes <= self.cache_bytes:
                 self._cache[key] = G
+        return G, stream
+
+    def start(self, req, slot):
+        G, stream = self._geometry(req, slot)
         job = _Job(req, stream, graph=self.graph, geometry=G)
         self._evict()
         return _Released(job, self)
+
+    def start_batch(self, reqs, slots):
+        """One batched job for requests of one batch_key; one handle each."""
+        keys = {self.batch_key(r) for r in reqs}
+        if len(keys) != 1 or None in keys or len(reqs) > self.max_batch:
+            raise ValueError("start_batch takes up to max_batch requests of one batch key")
+        G, stream = self._geometry(reqs[0], slots[0])
+        job = _BatchJob(reqs, stream, graph=self.graph, geometry=G)
+        self._evict()
+        return [_Released(job.member(j), self) for j in range(len(reqs))]
 
 
 class _Released:

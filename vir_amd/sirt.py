@@ -88,15 +88,28 @@ class SIRT:
 
     W_sino / W_img default to A 1 and A^T 1.  step() runs one iteration in
     place on self.x; run(n) runs n; graph=True captures one iteration into a
-    HIP graph on first use and replays it."""
+    HIP graph on first use and replays it.
 
-    def __init__(self, P, b, x0=None, W_sino=None, W_img=None, gamma=1.0, graph=False, fused=True):
+    nbatch > 1 runs that many reconstructions on the same projector at once
+    (b and x0 with a leading batch dimension, shared W_sino / W_img) through
+    the projector's batched fused steps (fp_residual_batch /
+    bp_update_batch): each projection traces the rays once for a group of
+    items, and every item's iterates equal those of its own SIRT, bit for
+    bit."""
+
+    def __init__(self, P, b, x0=None, W_sino=None, W_img=None, gamma=1.0, graph=False, fused=True, nbatch=1):
         """fused: use the projector's fused steps when it has them."""
         require_gpu()
         self.P = P
+        self.nbatch = int(nbatch)
+        if self.nbatch < 1:
+            raise ValueError("nbatch must be >= 1")
         self.fused = bool(fused) and bool(getattr(P, "fused_sirt", False))
+        if self.nbatch > 1 and not (self.fused and hasattr(P, "fp_residual_batch")):
+            raise ValueError("a batched SIRT needs a projector with batched fused steps")
         self.img_shape, self.sino_shape = _shapes(P)
-        self.b = as_device_f32(b).reshape(self.sino_shape).contiguous()
+        lead = (self.nbatch,) if self.nbatch > 1 else ()
+        self.b = as_device_f32(b).reshape(lead + self.sino_shape).contiguous()
         dev = self.b.device
         if W_sino is None:
             W_sino = P.fp(torch.ones(self.img_shape, dtype=torch.float32, device=dev))
@@ -104,16 +117,20 @@ class SIRT:
             W_img = P.bp(torch.ones(self.sino_shape, dtype=torch.float32, device=dev))
         self.W_sino = as_device_f32(W_sino).reshape(self.sino_shape).contiguous()
         self.W_img = as_device_f32(W_img).reshape(self.img_shape).contiguous()
-        self.x = (torch.zeros(self.img_shape, dtype=torch.float32, device=dev) if x0 is None
-                  else as_device_f32(x0).reshape(self.img_shape).clone())
-        self.r = torch.empty(self.sino_shape, dtype=torch.float32, device=dev)
-        self.u = torch.empty(self.img_shape, dtype=torch.float32, device=dev)
+        self.x = (torch.zeros(lead + self.img_shape, dtype=torch.float32, device=dev) if x0 is None
+                  else as_device_f32(x0).reshape(lead + self.img_shape).clone())
+        self.r = torch.empty(lead + self.sino_shape, dtype=torch.float32, device=dev)
+        self.u = torch.empty(self.img_shape, dtype=torch.float32, device=dev) if self.nbatch == 1 else None
         self.gamma = float(gamma)
         self.use_graph = graph
         self._graph = None
         self.iterations = 0
 
     def _iteration(self):
+        if self.nbatch > 1:
+            self.P.fp_residual_batch(self.x, self.b, self.W_sino, self.r)
+            self.P.bp_update_batch(self.r, self.W_img, self.gamma, self.x)
+            return
         if self.fused:
             self.P.fp_residual(self.x, self.b, self.W_sino, self.r)
             self.P.bp_update(self.r, self.W_img, self.gamma, self.x)
