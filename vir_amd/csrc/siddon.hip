@@ -434,11 +434,12 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
                     if (miss || alo > ahi) continue;
                     // exact plane slices only matter near the ray's ends
                     if (NB == 1 && !rec && !window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
+                    const bool mid = UNG && window_mid(g, R, alo, ahi);
                     walk_dispatch<UNG>(g, R, [&](auto fast) {
                         constexpr bool F = decltype(fast)::value;
                         Walker W;
                         walker_seek<F>(g, R, W, alo);
-                        walk_entries<P2, ISO, false, F>(g, R, W, ahi, [&](int ix, int iy, int iz, float len) {
+                        auto visit = [&](int ix, int iy, int iz, float len) {
                             int lx = ix - b0[0], ly = iy - b0[1], lz = iz - b0[2];
                             if ((unsigned)lx < (unsigned)(b1[0] - b0[0]) &&
                                 (unsigned)ly < (unsigned)(b1[1] - b0[1]) &&
@@ -447,7 +448,19 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
 #pragma unroll
                                 for (int j = 0; j < NB; ++j) acc[j * NS * kVox + li] += wgt[j] * len;
                             }
-                        });
+                        };
+                        // windows away from the rays' ends: the walk without
+                        // the special values (same entries) when every
+                        // walking lane's window qualifies (in the kernels
+                        // for unguarded ray sets; the guarded ones keep their
+                        // registers for the float64 and range-checked paths)
+                        if constexpr (F && UNG) {
+                            if (__ballot(!mid) == 0) {
+                                walk_mid<P2, ISO>(g, R, W, ahi, visit);
+                                return;
+                            }
+                        }
+                        walk_entries<P2, ISO, false, F>(g, R, W, ahi, visit);
                     });
                 }
                 // next colour phase: keep the compiler from moving LDS accesses

@@ -639,6 +639,77 @@ __device__ __forceinline__ void walk_entries(const GridK& g, const Ray& R, Walke
     }
 }
 
+// True when neither special value (alpha_min, alpha_max) can be popped by
+// a walk over [t0, t1] entered with walker_seek(t0), and no end trim
+// applies: along the axis crossing the most planes per unit alpha, at least
+// 2.5 planes lie between alpha_min and t0 and 3.5 between t1 and
+// alpha_max, so walker_seek consumes alpha_min and at least one plane, and
+// the element that ends the walk (the first past t1) is a plane before
+// alpha_max (the exact slices differ from the geometric entry and exit by
+// at most one plane; the half plane covers float rounding).  NaN fails.
+__device__ __forceinline__ bool window_mid(const GridK& g, const Ray& R, float t0, float t1) {
+    float gmax = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gmax = fmaxf(gmax, fabsf(R.dST[a]) * g.inv_d[a]);
+    return (t0 - R.amin) * gmax >= 2.5f && (R.amax - t1) * gmax >= 3.5f;
+}
+
+// walker_pop<ISO, true> for a walk over a window_mid window: alpha_max is
+// never the smallest pending element and the element count never reaches
+// its end, so the special-value pick and the count are left out; streams
+// that run out still yield +inf.  Same elements as walker_pop there.
+template <bool ISO>
+__device__ __forceinline__ float walker_pop_mid(const GridK& g, const Ray& R, Walker& W) {
+    const float h0 = W.h0, h1 = W.h1, h2 = W.h2;
+    const bool c1 = h1 < h0;
+    float hm = c1 ? h1 : h0;
+    const bool c2 = h2 < hm;
+    hm = c2 ? h2 : hm;
+    const bool s2 = c2, s1 = !c2 && c1, s0 = !c2 && !c1;
+    // (picks among values, not among members: a conditional over lvalues
+    // would pick an address)
+    const float ln0 = W.ln0, ln1 = W.ln1, ln2 = W.ln2;
+    const float st0 = W.lstep0, st1 = W.lstep1, st2 = W.lstep2;
+    const float ls0 = W.lstop0, ls1 = W.lstop1, ls2 = W.lstop2;
+    const float l = (c2 ? ln2 : (c1 ? ln1 : ln0)) + (c2 ? st2 : (c1 ? st1 : st0));
+    const float ls = c2 ? ls2 : (c1 ? ls1 : ls0);
+    float pc;
+    if (ISO) {
+        pc = plane_lin(g.d[0], g.o[0], l);
+    } else {
+        pc = plane_lin(c2 ? g.d[2] : (c1 ? g.d[1] : g.d[0]), c2 ? g.o[2] : (c1 ? g.o[1] : g.o[0]), l);
+    }
+    const float ws0 = R.s[0], ws1 = R.s[1], ws2 = R.s[2];
+    const float we0 = R.dST[0], we1 = R.dST[1], we2 = R.dST[2];
+    const float wy0 = R.y[0], wy1 = R.y[1], wy2 = R.y[2];
+    const float sa = c2 ? ws2 : (c1 ? ws1 : ws0);
+    const float da = c2 ? we2 : (c1 ? we1 : we0);
+    const float ya = c2 ? wy2 : (c1 ? wy1 : wy0);
+    float hn = div_rn_unguarded(pc - sa, da, ya);
+    hn = l != ls ? hn : VIR_INF;
+    W.ln0 = s0 ? l : ln0;
+    W.ln1 = s1 ? l : ln1;
+    W.ln2 = s2 ? l : ln2;
+    W.h0 = s0 ? hn : h0;
+    W.h1 = s1 ? hn : h1;
+    W.h2 = s2 ? hn : h2;
+    return hm;
+}
+
+// walk_entries<P2, ISO, false, true> over a window_mid window after
+// walker_seek to its start: every pair is a middle pair (no end trims) and
+// the walk stops on a_stop alone.
+template <bool P2, bool ISO, class Visit>
+__device__ __forceinline__ void walk_mid(const GridK& g, const Ray& R, Walker& W, float a_stop, Visit&& visit) {
+    int idx[3];
+    float len;
+    while (!(W.prev > a_stop)) {
+        const float b = walker_pop_mid<ISO>(g, R, W);
+        if (segment<P2, false, true>(g, R, W.prev, b, idx, len)) visit(idx[0], idx[1], idx[2], len);
+        W.prev = b;
+    }
+}
+
 template <bool B>
 struct Fast {
     static constexpr bool value = B;
