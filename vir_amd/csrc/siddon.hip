@@ -154,9 +154,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(NB 
         Walker W;
         walker_begin(g, R, W);
         const unsigned nyz = (unsigned)(g.n[1] * g.n[2]);
-        walk_dispatch<UNG>(g, R, [&](auto fast) {
-            walk_entries<P2, ISO, false, decltype(fast)::value, true>(
-                g, R, W, VIR_INF, [&](int ix, int iy, int iz, float len, bool keep) {
+        auto visit = [&](int ix, int iy, int iz, float len, bool keep) {
                     bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
                               (unsigned)iz < (unsigned)g.n[2];
                     bad += (keep && !in) ? 1 : 0;
@@ -172,7 +170,24 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(NB 
                     }
                     pin = use;
                     pl = use ? len : 0.0f;
-                });
+        };
+        walk_dispatch<UNG>(g, R, [&](auto fast) {
+            constexpr bool F = decltype(fast)::value;
+            if constexpr (F && UNG) {
+                // the general walk to a few planes past the entry, the walk
+                // without the special values through the ray's middle (same
+                // entries; window_mid holds there), the general walk to the end
+                float gmax = 0.0f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) gmax = fmaxf(gmax, fabsf(R.dST[a]) * g.inv_d[a]);
+                const float a1 = R.amin + 3.0f / gmax, a2 = R.amax - 4.0f / gmax;
+                walk_entries<P2, ISO, false, true, true>(g, R, W, a1, visit);
+                if (W.consumed == 2 && window_mid(g, R, W.prev, a2))
+                    walk_mid<P2, ISO, true, true>(g, R, W, a2, visit);
+                walk_entries<P2, ISO, false, true, true>(g, R, W, VIR_INF, visit);
+            } else {
+                walk_entries<P2, ISO, false, F, true>(g, R, W, VIR_INF, visit);
+            }
         });
     }
 #pragma unroll

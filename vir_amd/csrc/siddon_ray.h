@@ -696,16 +696,20 @@ __device__ __forceinline__ float walker_pop_mid(const GridK& g, const Ray& R, Wa
     return hm;
 }
 
-// walk_entries<P2, ISO, false, true> over a window_mid window after
-// walker_seek to its start: every pair is a middle pair (no end trims) and
-// the walk stops on a_stop alone.
-template <bool P2, bool ISO, class Visit>
+// walk_entries<P2, ISO, false, true, ALL> over a window_mid window
+// [W.prev, a_stop], with the walker past alpha_min and its first pair:
+// every pair is a middle pair (no end trims) and the walk stops on a_stop
+// alone.  COUNT keeps W.remaining exact, for a walk_entries that carries on
+// from where this one stops.
+template <bool P2, bool ISO, bool ALL = false, bool COUNT = false, class Visit>
 __device__ __forceinline__ void walk_mid(const GridK& g, const Ray& R, Walker& W, float a_stop, Visit&& visit) {
     int idx[3];
     float len;
     while (!(W.prev > a_stop)) {
         const float b = walker_pop_mid<ISO>(g, R, W);
-        if (segment<P2, false, true>(g, R, W.prev, b, idx, len)) visit(idx[0], idx[1], idx[2], len);
+        if (COUNT) W.remaining -= 1;
+        const bool keep = segment<P2, false, true>(g, R, W.prev, b, idx, len);
+        visit_pair<ALL>(visit, idx, len, keep);
         W.prev = b;
     }
 }
