@@ -176,14 +176,16 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(NB 
             if constexpr (F && UNG) {
                 // the general walk to a few planes past the entry, the walk
                 // without the special values through the ray's middle (same
-                // entries; window_mid holds there), the general walk to the end
+                // entries; window_mid holds there; on power-of-two grids with
+                // indices from voxel-unit coordinates), the general walk to
+                // the end
                 float gmax = 0.0f;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) gmax = fmaxf(gmax, fabsf(R.dST[a]) * g.inv_d[a]);
                 const float a1 = R.amin + 3.0f / gmax, a2 = R.amax - 4.0f / gmax;
                 walk_entries<P2, ISO, false, true, true>(g, R, W, a1, visit);
-                if (W.consumed == 2 && window_mid(g, R, W.prev, a2))
-                    walk_mid<P2, ISO, true, true>(g, R, W, a2, visit);
+                if (W.consumed == 2 && window_mid(g, R, W.prev, a2) && (!P2 || scaled_exact(g, R)))
+                    walk_mid<P2, ISO, true, true, P2>(g, R, W, a2, visit);
                 walk_entries<P2, ISO, false, true, true>(g, R, W, VIR_INF, visit);
             } else {
                 walk_entries<P2, ISO, false, F, true>(g, R, W, VIR_INF, visit);

@@ -42,6 +42,11 @@ struct GridK {
     float eps_f;      // epsilon as numpy uses it against float32 operands
     float inv_eps;    // RN(1/eps_f)
     double eps_d;     // epsilon against float64 operands
+    // Coordinates in voxel units for the lean walk (see scaled_exact):
+    // scaled = 1 when every d is 2^k with |k| <= 20 and every p0 is 0 or
+    // of magnitude in [2^-80, 2^100]; then p0s = p0 / d, exactly.
+    int scaled;
+    float p0s[3];
 };
 
 // Correctly rounded a/b from y = RN(1/b) (Markstein): q = RN(a*y),
@@ -696,19 +701,72 @@ __device__ __forceinline__ float walker_pop_mid(const GridK& g, const Ray& R, Wa
     return hm;
 }
 
+// The ray in voxel units: s / d and dST / d per axis.
+struct ScaledRay {
+    f3v s, e;
+};
+
+// True when segment_scaled on this ray returns segment<true, false, true>'s
+// bits for every pair of alphas >= R.amin: on a g.scaled grid, scaling by
+// 1/d = 2^-k (|k| <= 20) commutes with each rounding of the index
+// computation as long as every operand and result stays in the normal
+// range before and after scaling.  That holds when each s is 0 or of
+// magnitude in [2^-80, 2^100] and each dST is 0 or has |dST| amin >= 2^-80
+// and |dST| <= 2^100 (with p0 likewise, g.scaled): every m dST is then 0 or
+// of magnitude >= 2^-80, every sum or difference of two such values is 0 or
+// a multiple of 2^-103 (so >= 2^-103), and nothing exceeds 2^102; scaled,
+// the bounds are 2^-123 and 2^122.
+__device__ __forceinline__ bool scaled_exact(const GridK& g, const Ray& R) {
+    bool ok = g.scaled != 0 && R.amin > 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float sa = fabsf(R.s[a]), ea = fabsf(R.dST[a]);
+        ok = ok && (sa == 0.0f || (sa >= 0x1p-80f && sa <= 0x1p100f));
+        ok = ok && (ea == 0.0f || (ea * R.amin >= 0x1p-80f && ea <= 0x1p100f));
+    }
+    return ok;
+}
+
+__device__ __forceinline__ ScaledRay scale_ray(const GridK& g, const Ray& R) {
+    ScaledRay S;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        S.s[a] = R.s[a] * g.inv_d[a];
+        S.e[a] = R.dST[a] * g.inv_d[a];
+    }
+    return S;
+}
+
+// segment<true, false, true> from the voxel-unit ray: the per-axis index
+// (s + m dST - p0) / d computed as s' + m e' - p0s, one multiplication
+// fewer per axis; same bits where scaled_exact holds.
+__device__ __forceinline__ bool segment_scaled(const GridK& g, const Ray& R, const ScaledRay& S, float a0, float a1,
+                                               int idx[3], float& len) {
+    const float dA = a1 - a0;
+    const float m = 0.5f * (a1 + a0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) idx[a] = (int)((S.s[a] + m * S.e[a]) - g.p0s[a]);
+    len = R.dist * dA;
+    return len > 0.0f;
+}
+
 // walk_entries<P2, ISO, false, true, ALL> over a window_mid window
 // [W.prev, a_stop], with the walker past alpha_min and its first pair:
 // every pair is a middle pair (no end trims) and the walk stops on a_stop
 // alone.  COUNT keeps W.remaining exact, for a walk_entries that carries on
-// from where this one stops.
-template <bool P2, bool ISO, bool ALL = false, bool COUNT = false, class Visit>
+// from where this one stops.  SCALED: indices by segment_scaled (P2 grids,
+// rays for which scaled_exact holds).
+template <bool P2, bool ISO, bool ALL = false, bool COUNT = false, bool SCALED = false, class Visit>
 __device__ __forceinline__ void walk_mid(const GridK& g, const Ray& R, Walker& W, float a_stop, Visit&& visit) {
     int idx[3];
     float len;
+    ScaledRay S;
+    if (SCALED) S = scale_ray(g, R);
     while (!(W.prev > a_stop)) {
         const float b = walker_pop_mid<ISO>(g, R, W);
         if (COUNT) W.remaining -= 1;
-        const bool keep = segment<P2, false, true>(g, R, W.prev, b, idx, len);
+        const bool keep = SCALED ? segment_scaled(g, R, S, W.prev, b, idx, len)
+                                 : segment<P2, false, true>(g, R, W.prev, b, idx, len);
         visit_pair<ALL>(visit, idx, len, keep);
         W.prev = b;
     }
