@@ -84,7 +84,11 @@ int vir_siddon_fill(const float *src, const float *trg, int64_t nrays,
 /* Adds to *count (one device int32, initialised by the caller) the number of
  * rays whose grid entry/exit division needs the range check, or that meet
  * the grid and need the range-checked plane division or the float64 alpha
- * list (a zero direction component) while tracing.  When the
+ * list (a zero direction component) while tracing, and, on grids whose
+ * voxel sizes are powers of two from 1/16 to 16 (walked in voxel units by
+ * the unguarded kernels), rays with a coordinate or direction component too
+ * small or too large for that walk to round as the world-unit one (near the
+ * bounds of the float range only).  When the
  * count is 0, vir_siddon_fp / vir_siddon_bp calls on the same rays and grid
  * may pass VIR_UNGUARDED: their kernels are then built without those paths
  * (same bits, fewer registers).  Passing the flag for other rays gives

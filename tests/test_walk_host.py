@@ -2,13 +2,16 @@
 
 tests/host/walk_check.cpp compiles the device functions of
 vir_amd/csrc/siddon_ray.h for the host (clang++, as hipcc's host pass would)
-and compares, pair by pair, the walks the kernels take away from a ray's
-ends -- the forward projector's three-phase walk and the backprojector's
-walk over a sub-brick window -- with the general walk: cone rays of the
-256^3 benchmark scan, the GPU tests' cone geometry, a non-power-of-two grid,
-4 mm and 0.25 mm voxels with an offset origin, and sources near the grid.
-With the margins of the lean walks dropped (a planted mistake) the same
-check must find mismatches."""
+and compares, pair by pair, the walks the unguarded kernels take -- the
+forward projector's three-phase walk and the backprojector's walk over a
+sub-brick window, without the special values away from a ray's ends and in
+voxel units on power-of-two grids -- with the general walk in world units:
+cone rays of the 256^3 benchmark scan, the GPU tests' cone geometry, a
+non-power-of-two grid, 4 mm and 0.25 mm voxels with an offset origin, and
+sources near the grid.
+With the margins of the lean walks dropped, or the reciprocal direction
+left in world units in the voxel-unit walk (planted mistakes), the same check
+must find mismatches."""
 import os
 import shutil
 import subprocess
@@ -85,14 +88,17 @@ def test_lean_walks_visit_the_general_walks_pairs(walk_check, tmp_path, name):
     _write(path, s, t, n, d, o)
     rc, c = _run(walk_check, path)
     assert rc == 0 and c["fp_mismatch"] == 0 and c["bp_mismatch"] == 0, c
-    # the lean walks carry most of the work, so the comparison covers them
+    # the lean walks carry most of the work, so the comparison covers them;
+    # every power-of-two grid here is walked in voxel units
     assert c["fp_mid_pairs"] > 0.5 * c["fp_pairs"], c
     assert c["bp_mid_windows"] > 0.3 * c["bp_windows"], c
+    assert c["unit_rays"] == (0 if name == "cone3d_d07" else c["rays"]), c
 
 
-def test_dropping_the_margins_is_caught(walk_check, tmp_path):
+@pytest.mark.parametrize("mistake", ["loose", "unscaled_y"])
+def test_planted_mistakes_are_caught(walk_check, tmp_path, mistake):
     s, t, n, d, o = _cases()["cone3d"]
     path = str(tmp_path / "cone3d.bin")
     _write(path, s, t, n, d, o)
-    rc, c = _run(walk_check, path, "loose")
+    rc, c = _run(walk_check, path, mistake)
     assert rc == 1 and c["fp_mismatch"] > 0 and c["bp_mismatch"] > 0, c

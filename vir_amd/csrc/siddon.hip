@@ -86,9 +86,10 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_fill(GridK g, const floa
 }
 
 // Rays whose entry/exit divisions need div_rn's range check
-// (entry_unguarded false), and rays that meet the grid and need it for a plane
-// division or take the float64 list (planes_unguarded false); one atomic per
-// wave.
+// (entry_unguarded false), rays that meet the grid and need it for a plane
+// division or take the float64 list (planes_unguarded false), and, on g.unit
+// grids (whose unguarded kernels walk in voxel units), rays for which that
+// walk is not exact (unit_exact false); one atomic per wave.
 __global__ __launch_bounds__(kRayThreads) void k_siddon_count_guarded(GridK g, const float* __restrict__ src,
                                                                      const float* __restrict__ trg,
                                                                      int64_t nrays, int32_t* __restrict__ count) {
@@ -97,7 +98,8 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_count_guarded(GridK g, c
     if (r < nrays) {
         Ray R;
         ray_setup_core(g, src + 3 * r, trg + 3 * r, R);
-        need = !entry_unguarded(g, R) || (R.hit && !planes_unguarded(g, R));
+        need = !entry_unguarded(g, R) || (R.hit && !planes_unguarded(g, R)) ||
+               (R.hit && g.unit && !unit_exact(g, R));
     }
     unsigned long long m = __ballot(need);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int)__popcll(m));
@@ -121,12 +123,15 @@ __global__ __launch_bounds__(kRayThreads) void k_siddon_ray_records(GridK g, con
 
 // NB volumes that share the rays (volume j at vol + j*vstride, its
 // projection at sino + j*nrays): one trace per ray serves all of them, and
-// each volume's sums are formed exactly as for NB = 1.
+// each volume's sums are formed exactly as for NB = 1.  Unguarded rays on
+// power-of-two grids are walked in voxel units (gu = unit_grid(g); the
+// host establishes unit_exact for every ray with the unguarded precondition).
 template <bool P2, bool ISO, bool UNG, int NB = 1>
 __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 7 : 6))) void k_siddon_fp(
     GridK g, const float* __restrict__ src, const float* __restrict__ trg, const float* __restrict__ rec,
     int64_t nrays, const float* __restrict__ vol, float* __restrict__ sino, int accumulate,
-    int32_t* __restrict__ err, SirtEpi E, int64_t vstride) {
+    int32_t* __restrict__ err, SirtEpi E, int64_t vstride, GridK gu) {
+    constexpr bool UNIT = P2 && UNG;
     int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
     int64_t r = blk * kRayThreads + threadIdx.x;
     if (r >= nrays) return;
@@ -151,8 +156,10 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(NB 
     float pl = 0.0f;
     bool pin = false;
     if (R.hit) {
+        if (UNIT) unit_ray(g, R);
+        const GridK& gw = UNIT ? gu : g;
         Walker W;
-        walker_begin(g, R, W);
+        walker_begin(gw, R, W);
         const unsigned nyz = (unsigned)(g.n[1] * g.n[2]);
         auto visit = [&](int ix, int iy, int iz, float len, bool keep) {
                     bool in = (unsigned)ix < (unsigned)g.n[0] && (unsigned)iy < (unsigned)g.n[1] &&
@@ -171,24 +178,22 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(NB 
                     pin = use;
                     pl = use ? len : 0.0f;
         };
-        walk_dispatch<UNG>(g, R, [&](auto fast) {
+        walk_dispatch<UNG>(gw, R, [&](auto fast) {
             constexpr bool F = decltype(fast)::value;
             if constexpr (F && UNG) {
                 // the general walk to a few planes past the entry, the walk
                 // without the special values through the ray's middle (same
-                // entries; window_mid holds there; on power-of-two grids with
-                // indices from voxel-unit coordinates), the general walk to
-                // the end
+                // entries; window_mid holds there), the general walk to the end
                 float gmax = 0.0f;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) gmax = fmaxf(gmax, fabsf(R.dST[a]) * g.inv_d[a]);
+                for (int a = 0; a < 3; ++a) gmax = fmaxf(gmax, fabsf(R.dST[a]) * gw.inv_d[a]);
                 const float a1 = R.amin + 3.0f / gmax, a2 = R.amax - 4.0f / gmax;
-                walk_entries<P2, ISO, false, true, true>(g, R, W, a1, visit);
-                if (W.consumed == 2 && window_mid(g, R, W.prev, a2) && (!P2 || scaled_exact(g, R)))
-                    walk_mid<P2, ISO, true, true, P2>(g, R, W, a2, visit);
-                walk_entries<P2, ISO, false, true, true>(g, R, W, VIR_INF, visit);
+                walk_entries<P2, ISO, false, true, true, UNIT>(gw, R, W, a1, visit);
+                if (W.consumed == 2 && window_mid(gw, R, W.prev, a2))
+                    walk_mid<P2, ISO, true, true, UNIT>(gw, R, W, a2, visit);
+                walk_entries<P2, ISO, false, true, true, UNIT>(gw, R, W, VIR_INF, visit);
             } else {
-                walk_entries<P2, ISO, false, F, true>(g, R, W, VIR_INF, visit);
+                walk_entries<P2, ISO, false, F, true>(gw, R, W, VIR_INF, visit);
             }
         });
     }
@@ -321,8 +326,11 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
     GridK g, const float* __restrict__ src, const float* __restrict__ trg, const float* __restrict__ rec,
     int nviews, int nu, int nv, const float* __restrict__ proj, const float* __restrict__ sino,
     float* __restrict__ out, int views_per_chunk, int nbx, int nby, int nbz, int64_t out_stride, SirtEpi E,
-    int64_t sstride, int64_t nvox) {
+    int64_t sstride, int64_t nvox, GridK gu) {
     constexpr int kVox = SBX * SBY * SBZ;
+    // unguarded rays on power-of-two grids are walked in voxel units (gu =
+    // unit_grid(g); unit_exact holds for every ray with the precondition)
+    constexpr bool UNIT = P2 && UNG;
     // NS views at a time per wave: lanes [slot*LPQ, (slot+1)*LPQ) work on view
     // va0 + slot and accumulate into accumulator copy `slot` of the
     // sub-brick (copies summed in slot order at the end), so rays of
@@ -452,10 +460,12 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
                     // exact plane slices only matter near the ray's ends
                     if (NB == 1 && !rec && !window_interior(g, R, alo, ahi)) ray_setup_slices(g, R);
                     const bool mid = UNG && window_mid(g, R, alo, ahi);
-                    walk_dispatch<UNG>(g, R, [&](auto fast) {
+                    if (UNIT) unit_ray(g, R);
+                    const GridK& gw = UNIT ? gu : g;
+                    walk_dispatch<UNG>(gw, R, [&](auto fast) {
                         constexpr bool F = decltype(fast)::value;
                         Walker W;
-                        walker_seek<F>(g, R, W, alo);
+                        walker_seek<F>(gw, R, W, alo);
                         auto visit = [&](int ix, int iy, int iz, float len) {
                             int lx = ix - b0[0], ly = iy - b0[1], lz = iz - b0[2];
                             if ((unsigned)lx < (unsigned)(b1[0] - b0[0]) &&
@@ -473,11 +483,11 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
                         // registers for the float64 and range-checked paths)
                         if constexpr (F && UNG) {
                             if (__ballot(!mid) == 0) {
-                                walk_mid<P2, ISO>(g, R, W, ahi, visit);
+                                walk_mid<P2, ISO, false, false, UNIT>(gw, R, W, ahi, visit);
                                 return;
                             }
                         }
-                        walk_entries<P2, ISO, false, F>(g, R, W, ahi, visit);
+                        walk_entries<P2, ISO, false, F, false, UNIT && F>(gw, R, W, ahi, visit);
                     });
                 }
                 // next colour phase: keep the compiler from moving LDS accesses
@@ -644,7 +654,7 @@ static void siddon_fp_launch(const GridK& g, const float* src, const float* trg,
     const int64_t nb = (nrays + kRayThreads - 1) / kRayThreads;
 #define VIR_FP_LAUNCH(P2_, ISO_, UNG_)                                                                       \
     hipLaunchKernelGGL((k_siddon_fp<P2_, ISO_, UNG_, NB>), dim3((unsigned)nb), dim3(kRayThreads), 0, st, g,  \
-                       src, trg, rec, nrays, vol, sino, acc, err_count, E, nvox)
+                       src, trg, rec, nrays, vol, sino, acc, err_count, E, nvox, unit_grid(g))
     if (ung) {
         if (p2 && iso) VIR_FP_LAUNCH(true, true, true);
         else if (p2) VIR_FP_LAUNCH(true, false, true);
@@ -677,7 +687,9 @@ static int siddon_fp(const float* src, const float* trg, const float* rec, int64
     if (nrays == 0) return VIR_OK;
     if (!vol || !sino) return fail_arg("vol/sino is NULL");
     int acc = (flags & VIR_ACCUMULATE) ? 1 : 0;
-    const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
+    const bool ung = (flags & VIR_UNGUARDED) != 0, iso = isotropic(g);
+    // unguarded power-of-two kernels walk in voxel units: g.unit grids only
+    const bool p2 = all_pow2(g) && (!ung || g.unit);
     hipStream_t st = as_stream(stream);
     for (int32_t j0 = 0; j0 < nbatch;) {
         const int32_t left = nbatch - j0;
@@ -767,7 +779,9 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
     }
     const int nb = p.nbx * p.nby * p.nbz;
     const dim3 grd((unsigned)nb, (unsigned)p.nchunks);
-    const bool p2 = all_pow2(g), iso = isotropic(g), ung = (flags & VIR_UNGUARDED) != 0;
+    const bool ung = (flags & VIR_UNGUARDED) != 0, iso = isotropic(g);
+    // unguarded power-of-two kernels walk in voxel units: g.unit grids only
+    const bool p2 = all_pow2(g) && (!ung || g.unit);
     for (int32_t j0 = 0; j0 < nbatch;) {
         const int NBg = nbatch - j0 >= kBpBatch ? kBpBatch : 1;
         const float* sino_g = sino + j0 * nrays;
@@ -798,8 +812,9 @@ static int siddon_bp(const float* src, const float* trg, const float* rec, int32
             else VIR_BP_LAUNCH1(SHAPE, false, 1);                                                               \
         }                                                                                                       \
     } while (0)
-#define VIR_BP_ARGS \
-    g, src, trg, rec, nviews, nu, nv, proj, sino_g, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, out_stride, Ek, nrays, nvox
+#define VIR_BP_ARGS                                                                                                   \
+    g, src, trg, rec, nviews, nu, nv, proj, sino_g, out, p.views_per_chunk, p.nbx, p.nby, p.nbz, out_stride, Ek, nrays, \
+        nvox, unit_grid(g)
 #define BP3_ARGS 16, 8, 8, 2, 2, 4, 2
 #define BP3_WX BpShape3::wx
 #define BP3_WY BpShape3::wy

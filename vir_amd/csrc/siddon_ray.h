@@ -42,11 +42,11 @@ struct GridK {
     float eps_f;      // epsilon as numpy uses it against float32 operands
     float inv_eps;    // RN(1/eps_f)
     double eps_d;     // epsilon against float64 operands
-    // Coordinates in voxel units for the lean walk (see scaled_exact):
-    // scaled = 1 when every d is 2^k with |k| <= 20 and every p0 is 0 or
-    // of magnitude in [2^-80, 2^100]; then p0s = p0 / d, exactly.
-    int scaled;
-    float p0s[3];
+    // unit = 1: every d is 2^k with |k| <= 4 and every o, p0 and pN is 0
+    // or of magnitude in [2^-60, 2^100], so that walks in voxel units
+    // (unit_grid, unit_ray) give the same bits for rays where unit_exact
+    // holds
+    int unit;
 };
 
 // Correctly rounded a/b from y = RN(1/b) (Markstein): q = RN(a*y),
@@ -98,6 +98,10 @@ __device__ __forceinline__ float plane_lin(float d, float o, float lin) {
     t = t * 0.5f;
     return t + o;
 }
+
+// plane_lin in voxel units (d = 1, o in voxel units): l / 2 is exact, so
+// one fused multiply-add rounds once, as the add of plane_lin does.
+__device__ __forceinline__ float plane_unit(float o, float lin) { return __builtin_fmaf(lin, 0.5f, o); }
 
 __device__ __forceinline__ float plane_coord(const GridK& g, int a, int k) {
     return plane_at(g.n[a], g.d[a], g.o[a], k);
@@ -420,7 +424,7 @@ __device__ __forceinline__ void walker_begin(const GridK& g, const Ray& R, Walke
 // Branch-free in the stream choice: one division per step whichever axis
 // advances.  ISO: all three axes share (n, d, origin), so the plane
 // coordinate needs no per-axis selection.
-template <bool ISO, bool FAST>
+template <bool ISO, bool FAST, bool UNIT = false>
 __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker& W) {
     float h0 = W.h0, h1 = W.h1, h2 = W.h2;
     bool c1 = h1 < h0;
@@ -448,7 +452,9 @@ __device__ __forceinline__ float walker_pop(const GridK& g, const Ray& R, Walker
     float l = (c2 ? ln2 : (c1 ? ln1 : ln0)) + (c2 ? st2 : (c1 ? st1 : st0));
     float ls = c2 ? ls2 : (c1 ? ls1 : ls0);
     float pc;
-    if (ISO) {
+    if (UNIT) {
+        pc = plane_unit(ISO ? g.o[0] : (c2 ? g.o[2] : (c1 ? g.o[1] : g.o[0])), l);
+    } else if (ISO) {
         pc = plane_lin(g.d[0], g.o[0], l);
     } else {
         pc = plane_lin(c2 ? g.d[2] : (c1 ? g.d[1] : g.d[0]), c2 ? g.o[2] : (c1 ? g.o[1] : g.o[0]), l);
@@ -556,7 +562,7 @@ __device__ __forceinline__ void walker_seek(const GridK& g, const Ray& R, Walker
 // siddon.py:796-809.  Returns false when the reference drops the entry
 // (length not > 0).  idx holds the truncated index per axis.  P2: every
 // voxel size is a power of two, so dividing by it is a multiplication.
-template <bool P2, bool EXACT, bool FAST = false>
+template <bool P2, bool EXACT, bool FAST = false, bool UNIT = false>
 __device__ __forceinline__ bool segment(const GridK& g, const Ray& R, float a0, float a1,
                                         int idx[3], float& len) {
     // idx: truncated index per axis.  EXACT maps values outside int32 to
@@ -580,7 +586,8 @@ __device__ __forceinline__ bool segment(const GridK& g, const Ray& R, float a0, 
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float pos = (R.s[a] + m * R.dST[a]) - g.p0[a];
-            float v = P2 ? pos * g.inv_d[a] : div_rn(pos, g.d[a], g.inv_d[a]);
+            // (in voxel units d = 1: the position is the index coordinate)
+            float v = UNIT ? pos : (P2 ? pos * g.inv_d[a] : div_rn(pos, g.d[a], g.inv_d[a]));
             idx[a] = EXACT ? to_i32_f(v) : (int)v;
         }
         len = R.dist * dA;
@@ -609,36 +616,37 @@ __device__ __forceinline__ void visit_pair(Visit& visit, const int idx[3], float
     else if (keep) visit(idx[0], idx[1], idx[2], len);
 }
 
-template <bool P2, bool ISO, bool EXACT = false, bool FAST = false, bool ALL = false, class Visit>
+template <bool P2, bool ISO, bool EXACT = false, bool FAST = false, bool ALL = false, bool UNIT = false,
+          class Visit>
 __device__ __forceinline__ void walk_entries(const GridK& g, const Ray& R, Walker& W,
                                              float a_stop, Visit&& visit) {
     int idx[3];
     float len;
     if (W.consumed == 0) {
         if (W.remaining == 0) return;
-        W.prev = walker_pop<ISO, FAST>(g, R, W);
+        W.prev = walker_pop<ISO, FAST, UNIT>(g, R, W);
         W.consumed = 1;
     }
     if (W.consumed == 1) {
         if (W.remaining == 0 || W.prev > a_stop) return;
-        float b = walker_pop<ISO, FAST>(g, R, W);
+        float b = walker_pop<ISO, FAST, UNIT>(g, R, W);
         bool skip = trim_close<FAST>(R, W.prev, b);                   // drop A[0]
         if (W.remaining == 0) skip |= trim_close<FAST>(R, b, W.prev);  // drop A[-1]
-        bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
+        bool keep = segment<P2, EXACT, FAST, UNIT>(g, R, W.prev, b, idx, len);
         visit_pair<ALL>(visit, idx, len, keep && !skip);
         W.prev = b;
         W.consumed = 2;
     }
     while (W.remaining > 1 && !(W.prev > a_stop)) {
-        float b = walker_pop<ISO, FAST>(g, R, W);
-        bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
+        float b = walker_pop<ISO, FAST, UNIT>(g, R, W);
+        bool keep = segment<P2, EXACT, FAST, UNIT>(g, R, W.prev, b, idx, len);
         visit_pair<ALL>(visit, idx, len, keep);
         W.prev = b;
     }
     if (W.remaining == 1 && !(W.prev > a_stop)) {
-        float b = walker_pop<ISO, FAST>(g, R, W);
+        float b = walker_pop<ISO, FAST, UNIT>(g, R, W);
         bool skip = trim_close<FAST>(R, b, W.prev);                    // drop A[-1]
-        bool keep = segment<P2, EXACT, FAST>(g, R, W.prev, b, idx, len);
+        bool keep = segment<P2, EXACT, FAST, UNIT>(g, R, W.prev, b, idx, len);
         visit_pair<ALL>(visit, idx, len, keep && !skip);
         W.prev = b;
     }
@@ -659,11 +667,60 @@ __device__ __forceinline__ bool window_mid(const GridK& g, const Ray& R, float t
     return (t0 - R.amin) * gmax >= 2.5f && (R.amax - t1) * gmax >= 3.5f;
 }
 
+// Walks in voxel units.  On a g.unit grid every d is 2^k, and dividing every
+// coordinate of axis a by d_a commutes with each rounding of the walk -- the
+// plane coordinates (l d/2 + o)/d = l/2 + o/d, the plane alphas (the ratio
+// of two scaled coordinates, its Markstein steps scaled by d and 1/d), the
+// midpoint positions and the index coordinates pos/d -- as long as every
+// operand and result stays normal before and after scaling; alphas, and so
+// the merged list, the trims and the lengths (dist stays in world units),
+// do not change.  unit_exact states that condition for a ray: on a g.unit
+// grid (|k| <= 4), each s is 0 or of magnitude in [2^-60, 2^100] and each
+// dST is 0 or has |dST| alpha_min >= 2^-59 and |dST| <= 2^100; with the
+// grid's coordinates likewise, every product m dST (m >= alpha_min) is at
+// least 2^-59 and every sum or difference of such values 0 or a multiple of
+// 2^-83, so nothing comes within 2^16 of the normal range's bottom (the
+// division's remainder, a multiple of about 2^-46 of its numerator, stays
+// on the subnormal grid), nor near its top.
+__host__ __device__ inline GridK unit_grid(const GridK& g) {
+    GridK u = g;
+    for (int a = 0; a < 3; ++a) {
+        u.d[a] = 1.0f;
+        u.inv_d[a] = 1.0f;
+        u.o[a] = g.o[a] * g.inv_d[a];
+        u.p0[a] = g.p0[a] * g.inv_d[a];
+        u.pN[a] = g.pN[a] * g.inv_d[a];
+        u.pow2[a] = 1;
+    }
+    return u;
+}
+
+__device__ __forceinline__ bool unit_exact(const GridK& g, const Ray& R) {
+    bool ok = g.unit != 0 && R.amin > 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float sa = fabsf(R.s[a]), ea = fabsf(R.dST[a]);
+        ok = ok && (sa == 0.0f || (sa >= 0x1p-60f && sa <= 0x1p100f));
+        ok = ok && (ea == 0.0f || (ea * R.amin >= 0x1p-59f && ea <= 0x1p100f));
+    }
+    return ok;
+}
+
+// The ray's coordinates in voxel units: s / d, dST / d and y d = RN(1/(dST/d)).
+__device__ __forceinline__ void unit_ray(const GridK& g, Ray& R) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        R.s[a] = R.s[a] * g.inv_d[a];
+        R.dST[a] = R.dST[a] * g.inv_d[a];
+        R.y[a] = R.y[a] * g.d[a];
+    }
+}
+
 // walker_pop<ISO, true> for a walk over a window_mid window: alpha_max is
 // never the smallest pending element and the element count never reaches
 // its end, so the special-value pick and the count are left out; streams
 // that run out still yield +inf.  Same elements as walker_pop there.
-template <bool ISO>
+template <bool ISO, bool UNIT = false>
 __device__ __forceinline__ float walker_pop_mid(const GridK& g, const Ray& R, Walker& W) {
     const float h0 = W.h0, h1 = W.h1, h2 = W.h2;
     const bool c1 = h1 < h0;
@@ -679,7 +736,9 @@ __device__ __forceinline__ float walker_pop_mid(const GridK& g, const Ray& R, Wa
     const float l = (c2 ? ln2 : (c1 ? ln1 : ln0)) + (c2 ? st2 : (c1 ? st1 : st0));
     const float ls = c2 ? ls2 : (c1 ? ls1 : ls0);
     float pc;
-    if (ISO) {
+    if (UNIT) {
+        pc = plane_unit(ISO ? g.o[0] : (c2 ? g.o[2] : (c1 ? g.o[1] : g.o[0])), l);
+    } else if (ISO) {
         pc = plane_lin(g.d[0], g.o[0], l);
     } else {
         pc = plane_lin(c2 ? g.d[2] : (c1 ? g.d[1] : g.d[0]), c2 ? g.o[2] : (c1 ? g.o[1] : g.o[0]), l);
@@ -701,72 +760,19 @@ __device__ __forceinline__ float walker_pop_mid(const GridK& g, const Ray& R, Wa
     return hm;
 }
 
-// The ray in voxel units: s / d and dST / d per axis.
-struct ScaledRay {
-    f3v s, e;
-};
-
-// True when segment_scaled on this ray returns segment<true, false, true>'s
-// bits for every pair of alphas >= R.amin: on a g.scaled grid, scaling by
-// 1/d = 2^-k (|k| <= 20) commutes with each rounding of the index
-// computation as long as every operand and result stays in the normal
-// range before and after scaling.  That holds when each s is 0 or of
-// magnitude in [2^-80, 2^100] and each dST is 0 or has |dST| amin >= 2^-80
-// and |dST| <= 2^100 (with p0 likewise, g.scaled): every m dST is then 0 or
-// of magnitude >= 2^-80, every sum or difference of two such values is 0 or
-// a multiple of 2^-103 (so >= 2^-103), and nothing exceeds 2^102; scaled,
-// the bounds are 2^-123 and 2^122.
-__device__ __forceinline__ bool scaled_exact(const GridK& g, const Ray& R) {
-    bool ok = g.scaled != 0 && R.amin > 0.0f;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float sa = fabsf(R.s[a]), ea = fabsf(R.dST[a]);
-        ok = ok && (sa == 0.0f || (sa >= 0x1p-80f && sa <= 0x1p100f));
-        ok = ok && (ea == 0.0f || (ea * R.amin >= 0x1p-80f && ea <= 0x1p100f));
-    }
-    return ok;
-}
-
-__device__ __forceinline__ ScaledRay scale_ray(const GridK& g, const Ray& R) {
-    ScaledRay S;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        S.s[a] = R.s[a] * g.inv_d[a];
-        S.e[a] = R.dST[a] * g.inv_d[a];
-    }
-    return S;
-}
-
-// segment<true, false, true> from the voxel-unit ray: the per-axis index
-// (s + m dST - p0) / d computed as s' + m e' - p0s, one multiplication
-// fewer per axis; same bits where scaled_exact holds.
-__device__ __forceinline__ bool segment_scaled(const GridK& g, const Ray& R, const ScaledRay& S, float a0, float a1,
-                                               int idx[3], float& len) {
-    const float dA = a1 - a0;
-    const float m = 0.5f * (a1 + a0);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) idx[a] = (int)((S.s[a] + m * S.e[a]) - g.p0s[a]);
-    len = R.dist * dA;
-    return len > 0.0f;
-}
-
 // walk_entries<P2, ISO, false, true, ALL> over a window_mid window
 // [W.prev, a_stop], with the walker past alpha_min and its first pair:
 // every pair is a middle pair (no end trims) and the walk stops on a_stop
 // alone.  COUNT keeps W.remaining exact, for a walk_entries that carries on
-// from where this one stops.  SCALED: indices by segment_scaled (P2 grids,
-// rays for which scaled_exact holds).
-template <bool P2, bool ISO, bool ALL = false, bool COUNT = false, bool SCALED = false, class Visit>
+// from where this one stops.
+template <bool P2, bool ISO, bool ALL = false, bool COUNT = false, bool UNIT = false, class Visit>
 __device__ __forceinline__ void walk_mid(const GridK& g, const Ray& R, Walker& W, float a_stop, Visit&& visit) {
     int idx[3];
     float len;
-    ScaledRay S;
-    if (SCALED) S = scale_ray(g, R);
     while (!(W.prev > a_stop)) {
-        const float b = walker_pop_mid<ISO>(g, R, W);
+        const float b = walker_pop_mid<ISO, UNIT>(g, R, W);
         if (COUNT) W.remaining -= 1;
-        const bool keep = SCALED ? segment_scaled(g, R, S, W.prev, b, idx, len)
-                                 : segment<P2, false, true>(g, R, W.prev, b, idx, len);
+        const bool keep = segment<P2, false, true, UNIT>(g, R, W.prev, b, idx, len);
         visit_pair<ALL>(visit, idx, len, keep);
         W.prev = b;
     }

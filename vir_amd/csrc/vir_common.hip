@@ -51,14 +51,15 @@ int make_grid(const vir_grid* gp, GridK& g) {
     g.eps_f = (float)gp->eps;
     g.inv_eps = 1.0f / g.eps_f;
     g.eps_d = gp->eps;
-    g.scaled = 1;
+    g.unit = 1;
     for (int a = 0; a < 3; ++a) {
         int ex;
         const float m = std::frexp(gp->d[a], &ex);  // d = 2^(ex-1) when m == 0.5
-        const float p0 = std::fabs(g.p0[a]);
-        if (!(m == 0.5f && ex - 1 >= -20 && ex - 1 <= 20)) g.scaled = 0;
-        if (!(p0 == 0.0f || (p0 >= 0x1p-80f && p0 <= 0x1p100f))) g.scaled = 0;
-        g.p0s[a] = g.p0[a] * g.inv_d[a];
+        if (!(m == 0.5f && ex - 1 >= -4 && ex - 1 <= 4)) g.unit = 0;
+        for (float c : {g.o[a], g.p0[a], g.pN[a]}) {
+            const float ac = std::fabs(c);
+            if (!(ac == 0.0f || (ac >= 0x1p-60f && ac <= 0x1p100f))) g.unit = 0;
+        }
     }
     return VIR_OK;
 }
