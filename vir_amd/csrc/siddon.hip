@@ -407,7 +407,8 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
                 // less than (q + 0.5)/mV 2^-23 < 0.5/mV while q < 2^22
                 const bool fq = M < (1 << 22);
                 const float inv_mV = 1.0f / (float)mV;
-                for (int q = lq; q < M; q += LPQ) {
+                // in-view offset of candidate q
+                auto ray_of = [&](int q) -> unsigned {
                     int qu, qv;
                     if (fq) {
                         qu = (int)(((float)q + 0.5f) * inv_mV);
@@ -416,17 +417,31 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
                         qu = q / mV;
                         qv = q % mV;
                     }
-                    int iu = us + qu * f.su;
-                    int iv = vs + qv * f.sv;
-                    const unsigned ro = __umul24((unsigned)iu, (unsigned)nv) + (unsigned)iv;
+                    const int iu = us + qu * f.su;
+                    const int iv = vs + qv * f.sv;
+                    return __umul24((unsigned)iu, (unsigned)nv) + (unsigned)iv;
+                };
+                // the lane's next candidate and its weights, loaded one
+                // candidate ahead so the load overlaps the current walk
+                unsigned ro_next = lq < M ? ray_of(lq) : 0u;
+                float w_next[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) w_next[j] = lq < M ? sino_v[j * sstride + ro_next] : 0.0f;
+                for (int q = lq; q < M; q += LPQ) {
+                    const unsigned ro = ro_next;
                     // zero-weight rays (they miss the object) skip the
                     // endpoint loads altogether
                     float wgt[NB];
                     bool live_w = false;
 #pragma unroll
                     for (int j = 0; j < NB; ++j) {
-                        wgt[j] = sino_v[j * sstride + ro];
+                        wgt[j] = w_next[j];
                         live_w |= wgt[j] != 0.0f;
+                    }
+                    if (q + LPQ < M) {
+                        ro_next = ray_of(q + LPQ);
+#pragma unroll
+                        for (int j = 0; j < NB; ++j) w_next[j] = sino_v[j * sstride + ro_next];
                     }
                     if (!live_w) continue;
                     Ray R;
