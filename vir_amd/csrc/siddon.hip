@@ -395,32 +395,45 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
         const float* __restrict__ rec_v = rec ? rec + kRayRec * vbase : nullptr;
         const int nU = f.iu1 - f.iu0 + 1, nV = f.iv1 - f.iv0 + 1;
         if (nU <= 0 || nV <= 0) continue;
+        // phase (cu, cv) holds the columns iu0 + du + k su, du = (cu - iu0 mod su)
+        // mod su, of which there are mU = qU + [du < rU] (nU = qU su + rU), and
+        // likewise the rows: per-pass constants, so a phase needs no division
+        const int ru = f.iu0 % f.su, qU = nU / f.su, rU = nU % f.su;
+        const int rv = f.iv0 % f.sv, qV = nV / f.sv, rV = nV % f.sv;
+        auto phase = [&](int cu, int cv, int& us, int& vs, int& mU, int& mV) {
+            int xu = cu - ru + f.su, xv = cv - rv + f.sv;
+            xu = xu >= f.su ? xu - f.su : xu;
+            xv = xv >= f.sv ? xv - f.sv : xv;
+            us = f.iu0 + xu;
+            vs = f.iv0 + xv;
+            mU = qU + (xu < rU ? 1 : 0);
+            mV = qV + (xv < rV ? 1 : 0);
+        };
+        // in-view offset of candidate q of a phase.  With fq (q < 2^21),
+        // q / mV is floor((q + 0.5) r) with r the hardware reciprocal of mV
+        // (within 1 ulp): the quotient's fraction lies in [0.5/mV, 1 - 0.5/mV]
+        // and the roundings move it by less than (q + 0.5)/mV 2^-22 < 0.5/mV
+        auto ray_at = [&](int us, int vs, int mV, float inv_mV, bool fq, int q) -> unsigned {
+            int qu, qv;
+            if (fq) {
+                qu = (int)(((float)q + 0.5f) * inv_mV);
+                qv = q - (int)__umul24((unsigned)qu, (unsigned)mV);
+            } else {
+                qu = q / mV;
+                qv = q % mV;
+            }
+            const int iu = us + qu * f.su;
+            const int iv = vs + qv * f.sv;
+            return __umul24((unsigned)iu, (unsigned)nv) + (unsigned)iv;
+        };
         for (int cu = 0; cu < f.su; ++cu) {
             for (int cv = 0; cv < f.sv; ++cv) {
-                int us = f.iu0 + ((cu - f.iu0 % f.su) + f.su) % f.su;
-                int vs = f.iv0 + ((cv - f.iv0 % f.sv) + f.sv) % f.sv;
-                int mU = us <= f.iu1 ? (f.iu1 - us) / f.su + 1 : 0;
-                int mV = vs <= f.iv1 ? (f.iv1 - vs) / f.sv + 1 : 0;
-                int M = mU * mV;
-                // q / mV as floor((q + 0.5) RN(1/mV)): the quotient's fraction
-                // lies in [0.5/mV, 1 - 0.5/mV] and the two roundings move it by
-                // less than (q + 0.5)/mV 2^-23 < 0.5/mV while q < 2^22
-                const bool fq = M < (1 << 22);
-                const float inv_mV = 1.0f / (float)mV;
-                // in-view offset of candidate q
-                auto ray_of = [&](int q) -> unsigned {
-                    int qu, qv;
-                    if (fq) {
-                        qu = (int)(((float)q + 0.5f) * inv_mV);
-                        qv = q - (int)__umul24((unsigned)qu, (unsigned)mV);
-                    } else {
-                        qu = q / mV;
-                        qv = q % mV;
-                    }
-                    const int iu = us + qu * f.su;
-                    const int iv = vs + qv * f.sv;
-                    return __umul24((unsigned)iu, (unsigned)nv) + (unsigned)iv;
-                };
+                int us, vs, mU, mV;
+                phase(cu, cv, us, vs, mU, mV);
+                const int M = mU * mV;
+                const bool fq = M < (1 << 21);
+                const float inv_mV = __builtin_amdgcn_rcpf((float)mV);
+                auto ray_of = [&](int q) -> unsigned { return ray_at(us, vs, mV, inv_mV, fq, q); };
                 // the lane's next candidate and its weights, loaded one
                 // candidate ahead so the load overlaps the current walk
                 unsigned ro_next = lq < M ? ray_of(lq) : 0u;
