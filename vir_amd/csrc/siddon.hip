@@ -226,7 +226,7 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
     w = P[8] * x + P[9] * y + P[10] * z + P[11];
 }
 
-// Footprint of the guard box [gb_lo, gb_hi] in view P on the detector,
+// Footprint of the guard box [gb_lo, gb_hi] on the detector in a view P,
 // computed by the whole wave (every lane must call it).
 //
 // Candidate rays: a ray with an entry in the sub-brick meets the guard
@@ -246,16 +246,37 @@ __device__ __forceinline__ void project(const float* P, float x, float y, float 
 // share a voxel; likewise in v.  The 5 % cover float rounding and
 // entries whose computed index is one voxel off their true midpoint by
 // ~1e-3 voxel.
-__device__ __forceinline__ ViewFoot view_foot(const GridK& g, const float* __restrict__ P, const float* gb_lo,
-                                              const float* gb_hi, int nu, int nv, int lane) {
+//
+// The NS views va0 .. va0 + NS - 1 of a pass are done together: corner lanes
+// 8 s .. 8 s + 7 project the guard box for view va0 + s and reduce within
+// their group, and each lane takes the results of its own slot's group and
+// view (views past the last one, nva, repeat it; their lanes are idle).  Per
+// view the operations and their order are those of one view alone.
+template <int NS>
+__device__ __forceinline__ ViewFoot view_foot(const GridK& g, const float* __restrict__ proj, int va0, int nva,
+                                              const float* gb_lo, const float* gb_hi, int nu, int nv, int lane,
+                                              int slot) {
+    static_assert(8 * NS <= 64, "eight corner lanes per view");
+    // element i of slot s's matrix: wave-uniform loads, picked per lane
+    auto pick = [&](int s, int i) -> float {
+        float x = proj[12 * va0 + i];
+#pragma unroll
+        for (int t = 1; t < NS; ++t) x = s == t ? proj[12 * (va0 + min(t, nva - 1)) + i] : x;
+        return x;
+    };
+    const int cs = lane >> 3;
+    const bool corner = cs < NS;
+    float Pc[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) Pc[i] = pick(cs, i);
     float gu = 0.f, gv = 0.f, gw = 0.f;
-    project(P, (lane & 4) ? gb_hi[0] : gb_lo[0], (lane & 2) ? gb_hi[1] : gb_lo[1],
+    project(Pc, (lane & 4) ? gb_hi[0] : gb_lo[0], (lane & 2) ? gb_hi[1] : gb_lo[1],
             (lane & 1) ? gb_hi[2] : gb_lo[2], gu, gv, gw);
-    int gok = (lane >= 8) || (gw > 0.0f);
-    float guu = lane < 8 ? gu / gw : VIR_INF, gvv = lane < 8 ? gv / gw : VIR_INF;
+    int gok = !corner || (gw > 0.0f);
+    float guu = corner ? gu / gw : VIR_INF, gvv = corner ? gv / gw : VIR_INF;
     float gumin = guu, gvmin = gvv;
-    float gumax = lane < 8 ? guu : -VIR_INF, gvmax = lane < 8 ? gvv : -VIR_INF;
-    float gwmin = lane < 8 ? gw : VIR_INF;
+    float gumax = corner ? guu : -VIR_INF, gvmax = corner ? gvv : -VIR_INF;
+    float gwmin = corner ? gw : VIR_INF;
 #pragma unroll
     for (int m = 1; m < 8; m <<= 1) {
         gumin = fminf(gumin, __shfl_xor(gumin, m));
@@ -265,11 +286,15 @@ __device__ __forceinline__ ViewFoot view_foot(const GridK& g, const float* __res
         gwmin = fminf(gwmin, __shfl_xor(gwmin, m));
         gok &= __shfl_xor(gok, m);
     }
-    // lane 0 holds the wave's results; make them wave-uniform
-    gumin = __shfl(gumin, 0); gumax = __shfl(gumax, 0);
-    gvmin = __shfl(gvmin, 0); gvmax = __shfl(gvmax, 0);
-    gwmin = __shfl(gwmin, 0);
-    gok = __shfl(gok, 0);
+    // the first lane of each group holds its view's results
+    const int src = 8 * slot;
+    gumin = __shfl(gumin, src); gumax = __shfl(gumax, src);
+    gvmin = __shfl(gvmin, src); gvmax = __shfl(gvmax, src);
+    gwmin = __shfl(gwmin, src);
+    gok = __shfl(gok, src);
+    float P[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) P[i] = pick(slot, i);
     float ku = 0.0f, kv = 0.0f;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -369,20 +394,13 @@ __global__ __launch_bounds__(64 * WX * WY) __attribute__((amdgpu_waves_per_eu(NB
     const int a_begin = blockIdx.y * views_per_chunk;
     const int a_end = live ? min(nviews, a_begin + views_per_chunk) : a_begin;
 
-    auto view_foot = [&](int va) -> ViewFoot {
-        return vir::view_foot(g, proj + 12 * va, gb_lo, gb_hi, nu, nv, lane);
-    };
-
     for (int va0 = a_begin; va0 < a_end; va0 += NS) {
         // each lane keeps the footprint of its slot's view; slots past the
         // chunk's last view get an empty one
-        ViewFoot f;
-        f.iu0 = 0; f.iu1 = -1; f.iv0 = 0; f.iv1 = -1; f.su = 1; f.sv = 1;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            if (va0 + s >= a_end) break;
-            const ViewFoot fs = view_foot(va0 + s);
-            if (s == slot) f = fs;
+        const int nva = min(NS, a_end - va0);
+        ViewFoot f = vir::view_foot<NS>(g, proj, va0, nva, gb_lo, gb_hi, nu, nv, lane, slot);
+        if (slot >= nva) {
+            f.iu0 = 0; f.iu1 = -1; f.iv0 = 0; f.iv1 = -1; f.su = 1; f.sv = 1;
         }
         const int va = min(va0 + slot, a_end - 1);
         // this view's rays: ray (va, iu, iv) at in-view offset iu*nv + iv
