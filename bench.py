@@ -44,6 +44,8 @@ def parse():
                     help="requests of the Poisson trace replayed by the serving section")
     ap.add_argument("--plain-requests", type=int, default=6,
                     help="requests run through the kernel-per-call path for comparison")
+    ap.add_argument("--plain-small", type=int, default=12,
+                    help="small requests (at most 128^2 / 128^3) compared with the per-call path")
     return ap.parse_args()
 
 
@@ -481,14 +483,15 @@ def batched_section(P, vol, fp_ms, bp_ms, steps=3):
     return res
 
 
-def serving_section(n_serve, n_plain):
+def serving_section(n_serve, n_plain, n_small):
     """The serving trace (Poisson arrivals at 200/s, 60 % fan-beam jobs of
     128..1024^2 and 180..720 views, 40 % cone-beam jobs of 64..256^3 and
     120..360 views, 20 SIRT iterations each): the first n_serve requests
     replayed in real time under a 200 GB budget, with the projector cache
     (32 GB) and same-geometry grouping on; the first n_plain requests also
     run through the kernel-per-call path and through a fresh server (empty
-    cache) with all arrivals at once."""
+    cache) with all arrivals at once, and so do the first n_small requests
+    of at most 128^2 / 128^3."""
     import dataclasses
     from vir_amd.serve import Server, SirtBackend, plain_sirt, poisson_trace, summarize
     import torch
@@ -528,6 +531,27 @@ def serving_section(n_serve, n_plain):
     out["per_call_path"] = {"requests": done, "jobs_per_s": done / plain_s,
                             "served_jobs_per_s_same_requests": served["jobs_per_s"],
                             "served_over_per_call": served["jobs_per_s"] / (done / plain_s)}
+    del srv2, res2
+    torch.cuda.empty_cache()
+    # the same comparison on the trace's small jobs (fan 128^2 and cone
+    # 64^3 / 128^3), whose per-call cost is host work rather than kernels
+    small = [dataclasses.replace(r, arrival=0.0) for r in trace if r.n <= 128][:n_small]
+    t0 = time.perf_counter()
+    done = 0
+    for r in small:
+        plain_sirt(r)
+        done += 1
+        if time.perf_counter() - t0 > 60.0:
+            break
+    plain_s = time.perf_counter() - t0
+    log(f"[serve] plain path: {done} small requests in {plain_s:.1f}s")
+    srv3 = Server(SirtBackend(n_streams=8, cache_bytes=0), 200 << 30, max_active=16)
+    res3 = srv3.run(small[:done], keep_results=False)
+    served = summarize(res3, srv3.elapsed)
+    out["per_call_path_small_jobs"] = {
+        "requests": done, "sizes": sorted({f"{r.kind} {r.n}" for r in small[:done]}),
+        "jobs_per_s": done / plain_s, "served_jobs_per_s_same_requests": served["jobs_per_s"],
+        "served_over_per_call": served["jobs_per_s"] / (done / plain_s)}
     return out
 
 
@@ -654,7 +678,7 @@ def main():
         torch.cuda.empty_cache()
         if not args.no_serve:
             log("[bench] serving section")
-            out["serving"] = serving_section(args.serve_requests, args.plain_requests)
+            out["serving"] = serving_section(args.serve_requests, args.plain_requests, args.plain_small)
     print(json.dumps(out), flush=True)
 
 

@@ -50,18 +50,20 @@ def circular_rays(DetsY, Views, geom="par", src_iso=None, det_iso=None, DetsZ=No
     src_iso, det_iso = _defaults(DetsY, geom, src_iso, det_iso)
     c, s = _trig(Views)
     shape = Views.shape + DetsY.shape
-    dz = None
     nv = 1
+    parts = [c, s, DetsY]
     if DetsZ is not None:
         DetsZ = np.asarray(DetsZ, np.float64).reshape(-1)
         shape = shape + DetsZ.shape
         nv = DetsZ.size
-        dz = torch.from_numpy(DetsZ).to(device)
+        parts.append(DetsZ)
+    # one upload for the cosines, sines and detector coordinates
+    dev = torch.from_numpy(np.concatenate(parts)).to(device)
+    na, ny = c.size, DetsY.size
+    cd, sd, dy = dev[:na], dev[na:2 * na], dev[2 * na:2 * na + ny]
+    dz = dev[2 * na + ny:] if DetsZ is not None else None
     src = torch.empty(shape + (3,), dtype=torch.float32, device=device)
     trg = torch.empty(shape + (3,), dtype=torch.float32, device=device)
-    cd = torch.from_numpy(c).to(device)
-    sd = torch.from_numpy(s).to(device)
-    dy = torch.from_numpy(DetsY).to(device)
     _lib.check(_lib.lib().vir_geom_circular(_MODES[geom], c.size, _lib.ptr(cd), _lib.ptr(sd),
                                             DetsY.size, _lib.ptr(dy), nv, _lib.ptr(dz), src_iso,
                                             det_iso, _lib.ptr(src), _lib.ptr(trg), stream()),

@@ -176,75 +176,90 @@ def view_projections(src, trg, tol=0.05):
     that for a world point X: iu = (P0.X~)/(P2.X~), iv = (P1.X~)/(P2.X~), the
     fractional ray index of the ray through X, with P2.X~ > 0 in front of the
     source.  Raises ValueError when a view does not fit the model within
-    `tol` ray-index units (every ray is checked)."""
+    `tol` ray-index units (every ray is checked).  Views are fitted in
+    groups of about 2^18 rays (arrays over the view axis)."""
+    nviews, nu, nv = trg.shape[:3]
+    step = max(1, (1 << 18) // max(1, nu * nv))
+    return np.concatenate([_fit_views(src[a:a + step], trg[a:a + step], tol, a)
+                           for a in range(0, nviews, step)], axis=0)
+
+
+def _fit_views(src, trg, tol, first):
+    """view_projections of a group of views; `first` is the index of the
+    group's first view (for the error messages)."""
     src = np.asarray(src, np.float64)
     trg = np.asarray(trg, np.float64)
     nviews, nu, nv = trg.shape[:3]
-    mats = np.zeros((nviews, 12), np.float64)
-    iu = np.arange(nu, dtype=np.float64)[:, None]
-    iv = np.arange(nv, dtype=np.float64)[None, :]
-    for a in range(nviews):
-        T = trg[a]
-        S = src[a]
-        T0 = T[0, 0]
-        Eu = (T[nu - 1, 0] - T0) / (nu - 1) if nu > 1 else None
-        Ev = (T[0, nv - 1] - T0) / (nv - 1) if nv > 1 else None
-        D0 = T0 - S[0, 0]
-        point = np.allclose(S, S[0, 0], rtol=0, atol=1e-6 * (1 + np.abs(S[0, 0]).max()))
-        if Eu is None:
-            # single detector column: any direction orthogonal to the ray
-            Eu = np.cross(D0, [0.0, 0.0, 1.0])
-            if np.linalg.norm(Eu) < 1e-12:
-                Eu = np.cross(D0, [0.0, 1.0, 0.0])
-            Eu = Eu / np.linalg.norm(Eu)
-        if Ev is None:
-            Ev = np.cross(D0, Eu)
-            Ev = Ev / np.linalg.norm(Ev)
-            flat_v = True
-        else:
-            flat_v = False
-        N = np.cross(Eu, Ev)
-        A = np.cross(Ev, N)   # u = (Q - T0).A / (Eu.A)
-        B = np.cross(N, Eu)   # v = (Q - T0).B / (Ev.B)
-        if point:
-            Sp = S[0, 0]
-            nd = N @ (T0 - Sp)
-            # u*w with w = N.(X - S): Q - T0 = (S - T0) + t (X - S), t = nd / N.(X-S)
-            cu = ((Sp - T0) @ A) * N + nd * A
-            cv = ((Sp - T0) @ B) * N + nd * B
-            sgn = 1.0 if nd > 0 else -1.0
-            ru = sgn * np.concatenate([cu, [-cu @ Sp]]) / (Eu @ A)
-            rv = sgn * np.concatenate([cv, [-cv @ Sp]]) / (Ev @ B)
-            rw = sgn * np.concatenate([N, [-N @ Sp]])
-        else:
-            Dir = D0
-            dirs = T - S
-            if not np.allclose(dirs, Dir, rtol=0, atol=1e-5 * (1 + np.abs(Dir).max())):
-                raise ValueError(f"view {a}: rays neither share a source nor a direction")
-            nd = N @ Dir
-            if abs(nd) < 1e-12:
-                raise ValueError(f"view {a}: rays parallel to the detector plane")
-            # Q = X + t Dir, t = N.(T0 - X)/nd; u = (Q - T0).A/(Eu.A)
-            # (Q - T0).A = (X - T0).A + (N.(T0 - X)/nd) Dir.A
-            gu = A - (Dir @ A) / nd * N
-            gv = B - (Dir @ B) / nd * N
-            ru = np.concatenate([gu, [-gu @ T0]]) / (Eu @ A)
-            rv = np.concatenate([gv, [-gv @ T0]]) / (Ev @ B)
-            rw = np.array([0.0, 0.0, 0.0, 1.0])
-        if flat_v:
-            rv = np.zeros(4)
-        M = np.stack([ru, rv, rw])
-        # every ray must project onto its own index
-        X = np.concatenate([T.reshape(-1, 3), np.ones((nu * nv, 1))], axis=1)
-        if not point:
-            X = np.concatenate([(0.5 * (S + T)).reshape(-1, 3), np.ones((nu * nv, 1))], axis=1)
-        h = X @ M.T
-        pu = (h[:, 0] / h[:, 2]).reshape(nu, nv)
-        pv = (h[:, 1] / h[:, 2]).reshape(nu, nv)
-        err = np.max(np.abs(pu - iu))
-        if not flat_v:
-            err = max(err, np.max(np.abs(pv - iv)))
-        if not np.isfinite(err) or err > tol:
-            raise ValueError(f"view {a}: detector is not an affine ray grid (err {err:.3g})")
-        mats[a] = M.reshape(-1)
-    return mats.astype(np.float32)
+
+    def dot(x, y):   # per-view dot products of (nviews, 3) rows
+        return (x[:, 0] * y[:, 0] + x[:, 1] * y[:, 1]) + x[:, 2] * y[:, 2]
+
+    def unit(x):
+        return x / np.linalg.norm(x, axis=1, keepdims=True)
+
+    T0 = trg[:, 0, 0]
+    S0 = src[:, 0, 0]
+    D0 = T0 - S0
+    atol = 1e-6 * (1 + np.abs(S0).max(axis=1))
+    point = np.all(np.abs(src - S0[:, None, None, :]) <= atol[:, None, None, None], axis=(1, 2, 3))
+    if nu > 1:
+        Eu = (trg[:, nu - 1, 0] - T0) / (nu - 1)
+    else:
+        # single detector column: any direction orthogonal to the ray
+        Eu = np.cross(D0, [0.0, 0.0, 1.0])
+        alt = np.cross(D0, [0.0, 1.0, 0.0])
+        Eu = unit(np.where((np.linalg.norm(Eu, axis=1) < 1e-12)[:, None], alt, Eu))
+    flat_v = nv == 1
+    Ev = unit(np.cross(D0, Eu)) if flat_v else (trg[:, 0, nv - 1] - T0) / (nv - 1)
+    N = np.cross(Eu, Ev)
+    A = np.cross(Ev, N)   # u = (Q - T0).A / (Eu.A)
+    B = np.cross(N, Eu)   # v = (Q - T0).B / (Ev.B)
+    euA, evB = dot(Eu, A), dot(Ev, B)
+    M = np.zeros((nviews, 3, 4), np.float64)
+    # point sources: u*w with w = N.(X - S): Q - T0 = (S - T0) + t (X - S), t = nd / N.(X-S)
+    P_ = np.nonzero(point)[0]
+    if P_.size:
+        Sp, T0p, Np, Ap, Bp = S0[P_], T0[P_], N[P_], A[P_], B[P_]
+        nd = dot(Np, T0p - Sp)
+        cu = dot(Sp - T0p, Ap)[:, None] * Np + nd[:, None] * Ap
+        cv = dot(Sp - T0p, Bp)[:, None] * Np + nd[:, None] * Bp
+        sgn = np.where(nd > 0, 1.0, -1.0)[:, None]
+        M[P_, 0] = sgn * np.concatenate([cu, -dot(cu, Sp)[:, None]], axis=1) / euA[P_, None]
+        M[P_, 1] = sgn * np.concatenate([cv, -dot(cv, Sp)[:, None]], axis=1) / evB[P_, None]
+        M[P_, 2] = sgn * np.concatenate([Np, -dot(Np, Sp)[:, None]], axis=1)
+    # parallel bundles: Q = X + t Dir, t = N.(T0 - X)/nd; u = (Q - T0).A/(Eu.A)
+    Q_ = np.nonzero(~point)[0]
+    if Q_.size:
+        Dir, T0q, Nq, Aq, Bq = D0[Q_], T0[Q_], N[Q_], A[Q_], B[Q_]
+        dirs = trg[Q_] - src[Q_]
+        dtol = 1e-5 * (1 + np.abs(Dir).max(axis=1))
+        same = np.all(np.abs(dirs - Dir[:, None, None, :]) <= dtol[:, None, None, None], axis=(1, 2, 3))
+        if not same.all():
+            raise ValueError(f"view {first + int(Q_[np.argmin(same)])}: rays neither share a source nor a "
+                             f"direction")
+        nd = dot(Nq, Dir)
+        if (np.abs(nd) < 1e-12).any():
+            raise ValueError(f"view {first + int(Q_[np.argmax(np.abs(nd) < 1e-12)])}: rays parallel to the "
+                             f"detector plane")
+        gu = Aq - (dot(Dir, Aq) / nd)[:, None] * Nq
+        gv = Bq - (dot(Dir, Bq) / nd)[:, None] * Nq
+        M[Q_, 0] = np.concatenate([gu, -dot(gu, T0q)[:, None]], axis=1) / euA[Q_, None]
+        M[Q_, 1] = np.concatenate([gv, -dot(gv, T0q)[:, None]], axis=1) / evB[Q_, None]
+        M[Q_, 2] = np.array([0.0, 0.0, 0.0, 1.0])
+    if flat_v:
+        M[:, 1] = 0.0
+    # every ray must project onto its own index (point sources: the target;
+    # parallel bundles: the midpoint)
+    X = np.where(point[:, None, None, None], trg, 0.5 * (src + trg)).reshape(nviews, nu * nv, 3)
+    h = np.matmul(X, M[:, :, :3].transpose(0, 2, 1)) + M[:, None, :, 3]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        pu = (h[..., 0] / h[..., 2]).reshape(nviews, nu, nv)
+        pv = (h[..., 1] / h[..., 2]).reshape(nviews, nu, nv)
+    err = np.abs(pu - np.arange(nu, dtype=np.float64)[None, :, None]).max(axis=(1, 2))
+    if not flat_v:
+        err = np.maximum(err, np.abs(pv - np.arange(nv, dtype=np.float64)[None, None, :]).max(axis=(1, 2)))
+    bad = ~(np.isfinite(err) & (err <= tol))
+    if bad.any():
+        a = int(np.nonzero(bad)[0][0])
+        raise ValueError(f"view {first + a}: detector is not an affine ray grid (err {err[a]:.3g})")
+    return M.reshape(nviews, 12).astype(np.float32)
