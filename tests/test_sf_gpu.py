@@ -109,9 +109,11 @@ def test_par3d_batched_matches_oracle_per_volume():
 
 def test_par3d_tiled_forward_equals_per_bin_gather():
     """The LDS-tiled parallel 3-D forward kernel gives the same bits as the
-    one-thread-per-bin gather (VIR_PER_OUTPUT): two bin tiles, two row
-    tiles, lines whose pixel band exceeds one staged chunk (du = 2 d), a
-    45-degree view, a batch of two.  Likewise the column backprojector
+    one-thread-per-bin gather (VIR_PER_OUTPUT): several bin tiles and a
+    partial one, two row tiles, lines whose pixel band exceeds one staged
+    chunk (du = 2 d), a 45-degree view, groups of four views with both
+    driving axes in one group and a last group of one view, a batch of two.
+    Likewise the column backprojector
     against the one-thread-per-voxel kernel (two slab tiles, 76 views: two
     view groups)."""
     import torch
@@ -120,7 +122,7 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     from vir_amd.sys_mat.pd import SFProjector
     rng = np.random.default_rng(21)
     shape, nu, nv = (200, 150, 64), 90, 70
-    ang = np.concatenate([np.linspace(0, np.pi, 11, endpoint=False), [np.pi / 4]])
+    ang = np.concatenate([np.linspace(0, np.pi, 11, endpoint=False), [np.pi / 4, 2.0]])
     P = SFProjector(shape, ang, nu, "par3d", nv=nv, du=1.0, dv=0.45, su=0.3, sv=-0.2, d_pix=0.5, batch=2)
     x = torch.from_numpy(rng.uniform(0, 1, (2,) + shape).astype(np.float32)).cuda()
     tiled = P.fp(x)

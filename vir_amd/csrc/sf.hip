@@ -287,18 +287,23 @@ __global__ __launch_bounds__(256) void k_sf_fp(SFGeo G, const float* __restrict_
 
 // ------------------------------------------ tiled parallel 3-D forward
 //
-// One workgroup computes a tile of kTU bins x kTV detector rows of one view
-// of one batch item; lane m of every wave owns detector row m_lo + m.  For
-// every orthogonal line, over the pixels whose footprints can meet the tile
-// (the union of the per-bin ranges of k_sf_fp):
+// One workgroup computes a tile of kTUB bins x kTV detector rows of kVG
+// neighbouring views of one batch item, one view per wave; lane m of every
+// wave owns detector row m_lo + m.  For every orthogonal line, over the
+// pixels whose footprints can meet the tile in any of the views (the union
+// of the per-bin ranges of k_sf_fp):
 //   P[k][m] = sum over the row's slabs q of w_q(m) * vox[k][first(m) + q]
 // read straight from the volume (lane m reads z = first(m) + q: consecutive
-// lanes, consecutive z) into LDS, while each wave lists, for its 16 bins,
-// the pixels whose footprints meet the bin and their weights; then
-//   acc[u][m] += wu(k, u) * P[k][m]
-// Weights, pixel order and arithmetic are those of k_sf_fp, so both give
-// the same bits.
-constexpr int kTU = 64;          // detector bins per tile (16 per wave)
+// lanes, consecutive z) into LDS -- P depends on the line and the pixel
+// only, so the views share it -- while each wave lists, for its view's
+// bins, the pixels whose footprints meet the bin and their weights; then
+//   acc[u][m] += wu(k, u) * P[k][m].
+// Views whose driving axis differs from the first view's are done in a
+// second pass over the lines.  Weights, pixel order and arithmetic are those
+// of k_sf_fp, so both give the same bits.
+constexpr int kTUB = 16;         // detector bins per tile and view
+constexpr int kVG = 4;           // views per workgroup (one per wave)
+constexpr int kTU = kTUB * kVG;  // footprint lists per workgroup
 constexpr int kTV = 64;          // detector rows per tile (one per lane)
 constexpr int kKMax = 128;       // pixels of one line handled at once
 constexpr int kWMax = 12;        // pixels whose footprint meets one bin (checked on the host)
@@ -306,11 +311,11 @@ constexpr int kWSlot = kWMax;    // footprint list stride per bin (LDS: 4 workgr
 constexpr int kWPer = 3;         // candidate pixels per lane: 4 lanes x 3 cover kWMax
 constexpr int kStageRows = 3;    // pixel rows whose volume loads are in flight together
 
-__global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
+__global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
                                                         const float* __restrict__ cs,
                                                         const float* __restrict__ sn,
                                                         const float* __restrict__ img, float* __restrict__ sino,
-                                                        int accumulate, int nut, int nmt) {
+                                                        int accumulate, int nut, int nmt, int nvg) {
     __shared__ float P[kKMax * kTV];
     // footprint list per bin: (LDS offset (k - kc) * kTV of the j-th pixel,
     // its weight overlap/du * amplitude as bits), read as one 64-bit word
@@ -323,15 +328,19 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
     blk /= nmt;
     const int ut = blk % nut;
     blk /= nut;
-    const int ia = blk % G.na;
-    const int b = blk / G.na;
-    const int u_lo = ut * kTU, u_n = min(kTU, G.nu - u_lo);
+    const int vg = blk % nvg;
+    const int b = blk / nvg;
+    const int u_lo = ut * kTUB, u_n = min(kTUB, G.nu - u_lo);
     const int m_lo = mt * kTV, m_n = min(kTV, G.nv - m_lo);
+    const int a0 = vg * kVG, nva = min(kVG, G.na - a0);
 
-    const SFView V = sf_view(G, cs[ia], sn[ia], ang[ia]);
+    // this wave's view
+    const int ia = a0 + wave;
+    const bool vwave = wave < nva;
+    const int iav = vwave ? ia : a0;
+    const SFView V = sf_view(G, cs[iav], sn[iav], ang[iav]);
     const float c = V.c, s = V.s;
     const bool xd = fabsf(s) >= fabsf(c);
-    const int nd = xd ? G.nx : G.ny, no = xd ? G.ny : G.nx;
     const float inv_d = 1.0f / G.d, inv_du = 1.0f / G.du;
     const float* vol = img + (int64_t)b * G.nx * G.ny * G.nz;
 
@@ -368,10 +377,11 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
     for (int m = 32; m >= 1; m >>= 1) nzs_max = max(nzs_max, __shfl_xor(nzs_max, m));
     const int nzs_u = __builtin_amdgcn_readfirstlane(nzs_max);
 
-    // footprint-list lanes: bin tu_w (this wave's 16 bins), candidate group grp
-    const int tu_w = wave * (kTU / 4) + (lane & 15), grp = lane >> 4;
-    const int iu_w = u_lo + tu_w;
-    const bool ubin = tu_w < u_n;
+    // footprint-list lanes: bin u_lo + (lane & 15) of this wave's view (list
+    // slot tu_w), candidate group grp
+    const int tu_w = wave * kTUB + (lane & 15), grp = lane >> 4;
+    const int iu_w = u_lo + (lane & 15);
+    const bool ubin = vwave && (lane & 15) < u_n;
     const float ul = sf_det(G.nu, G.du, G.su, iu_w), ur = sf_det(G.nu, G.du, G.su, iu_w + 1);
 
     // the tile's bin edges (footprint margin included)
@@ -384,134 +394,162 @@ __global__ __launch_bounds__(256) void k_sf_fp_par3_tile(SFGeo G, const float* _
     // of k_sf_fp.
     const float pd = xd ? -s : c, po = xd ? c : -s;
     const float inv_pd = 1.0f / pd, lin_b = po * inv_pd;
-    const float at0 = ut0 * inv_pd, at1 = ut1 * inv_pd;
     const float al = (ul - G.wmax) * inv_pd, ar = (ur + G.wmax) * inv_pd;
+    // every view of the group (wave-uniform): driving axis and the tile's
+    // band as at0 - oc lin_b .. at1 - oc lin_b
+    bool gxd[kVG];
+    float glin[kVG], gat0[kVG], gat1[kVG];
+    bool mixed = false;
+#pragma unroll
+    for (int w = 0; w < kVG; ++w) {
+        const int aw = a0 + min(w, nva - 1);
+        const float cw = cs[aw], sw = sn[aw];
+        gxd[w] = fabsf(sw) >= fabsf(cw);
+        const float pdw = gxd[w] ? -sw : cw, pow_ = gxd[w] ? cw : -sw;
+        const float ipd = 1.0f / pdw;
+        glin[w] = pow_ * ipd;
+        gat0[w] = ut0 * ipd;
+        gat1[w] = ut1 * ipd;
+        mixed |= w < nva && gxd[w] != gxd[0];
+    }
 
-    double acc[kTU / 4];
+    double acc[kTUB];
 #pragma unroll
-    for (int i = 0; i < kTU / 4; ++i) acc[i] = 0.0;
+    for (int i = 0; i < kTUB; ++i) acc[i] = 0.0;
 
-    for (int io = 0; io < no; ++io) {
-        const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
-        const float x0 = __builtin_fmaf(-oc, lin_b, at0);
-        const float x1 = __builtin_fmaf(-oc, lin_b, at1);
-        const float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
-        const float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
-        if (!(k0f < (float)nd + 1.0f) || !(k1f > -2.0f)) continue;
-        const int k0t = (int)fmaxf(floorf(k0f) - 1.0f, 0.0f);
-        const int k1t = (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1));
-        // this lane's bin: its candidate pixel range (as k_sf_fp)
-        int kb0 = 0, kb1 = -1;
-        if (ubin) {
-            const float y0 = __builtin_fmaf(-oc, lin_b, al);
-            const float y1 = __builtin_fmaf(-oc, lin_b, ar);
-            const float b0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
-            const float b1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
-            if (b0f < (float)nd + 1.0f && b1f > -2.0f) {
-                kb0 = (int)fmaxf(floorf(b0f) - 1.0f, 0.0f);
-                kb1 = (int)fminf(floorf(b1f) + 2.0f, (float)(nd - 1));
+    // views whose driving axis is that of the first view, then the others
+    for (int pass = 0; pass < (mixed ? 2 : 1); ++pass) {
+        const bool pxd = pass == 0 ? gxd[0] : !gxd[0];
+        const bool active = vwave && xd == pxd;
+        const int nd = pxd ? G.nx : G.ny, no = pxd ? G.ny : G.nx;
+        for (int io = 0; io < no; ++io) {
+            const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
+            // the union of the pass's views' pixel bands on this line
+            int k0t = nd, k1t = -1;
+    #pragma unroll
+            for (int w = 0; w < kVG; ++w) {
+                if (w >= nva || gxd[w] != pxd) continue;
+                const float x0 = __builtin_fmaf(-oc, glin[w], gat0[w]);
+                const float x1 = __builtin_fmaf(-oc, glin[w], gat1[w]);
+                const float k0f = fminf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
+                const float k1f = fmaxf(x0, x1) * inv_d + 0.5f * (float)nd - 0.5f;
+                if (!(k0f < (float)nd + 1.0f) || !(k1f > -2.0f)) continue;
+                k0t = min(k0t, (int)fmaxf(floorf(k0f) - 1.0f, 0.0f));
+                k1t = max(k1t, (int)fminf(floorf(k1f) + 2.0f, (float)(nd - 1)));
             }
-        }
-        for (int kc = k0t; kc <= k1t; kc += kKMax) {
-            const int nk = min(kKMax, k1t - kc + 1);
-            __syncthreads();  // the previous chunk's reads are done
-            // P rows of this wave's pixels, straight from the volume: the
-            // loads of kStageRows rows are in flight together, the first
-            // group's while the footprint lists below are built.  Every lane
-            // loads: sz[q] = 0 where q >= nzs and the row index is clamped
-            // to the chunk, both valid, and the sums skip those values.
-            float xs[kStageRows][kSlabs];
-            auto stage_load = [&](int kk0) {
-#pragma unroll
-                for (int r = 0; r < kStageRows; ++r) {
-                    const int kk = kk0 + 4 * r;
-                    const int k = kc + min(kk, nk - 1);
-                    const int ix = xd ? k : io, iy = xd ? io : k;
-                    const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
-#pragma unroll
-                    for (int q = 0; q < kSlabs; ++q) xs[r][q] = q < nzs_u ? col[sz[q]] : 0.0f;
+            if (k0t > k1t) continue;
+            // this lane's bin: its candidate pixel range (as k_sf_fp)
+            int kb0 = 0, kb1 = -1;
+            if (ubin && active) {
+                const float y0 = __builtin_fmaf(-oc, lin_b, al);
+                const float y1 = __builtin_fmaf(-oc, lin_b, ar);
+                const float b0f = fminf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
+                const float b1f = fmaxf(y0, y1) * inv_d + 0.5f * (float)nd - 0.5f;
+                if (b0f < (float)nd + 1.0f && b1f > -2.0f) {
+                    kb0 = (int)fmaxf(floorf(b0f) - 1.0f, 0.0f);
+                    kb1 = (int)fminf(floorf(b1f) + 2.0f, (float)(nd - 1));
                 }
-            };
-            auto stage_store = [&](int kk0) {
-#pragma unroll
-                for (int r = 0; r < kStageRows; ++r) {
-                    const int kk = kk0 + 4 * r;
-                    float sv_ = 0.0f;
-#pragma unroll
-                    for (int q = 0; q < kSlabs; ++q)
-                        if (q < nzs) sv_ += xs[r][q] * swt[q];
-                    if (kk < nk) P[kk * kTV + lane] = sv_;
-                }
-            };
-            stage_load(wave);
-            // footprint lists of this wave's bins over the chunk, in pixel
-            // order: lane group grp tests candidates ka + 3 grp .. ka + 3 grp + 2
-            {
-                const int ka = max(kb0, kc), kz = min(kb1, kc + nk - 1);
-                float wv[kWPer];
-                int nzc = 0;
-#pragma unroll
-                for (int j = 0; j < kWPer; ++j) {
-                    const int k = ka + grp * kWPer + j;
-                    float w = 0.0f;
-                    if (k <= kz) {
-                        const int ix = xd ? k : io, iy = xd ? io : k;
-                        const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
-                        const float ou = ovl(f.u0, f.u1, ul, ur);
-                        if (ou > 0.0f) w = (ou * inv_du) * f.amp;
-                    }
-                    wv[j] = w;
-                    nzc += (k <= kz && w != 0.0f) ? 1 : 0;
-                }
-                // exclusive prefix of the group counts of this bin (lanes l, l+16, l+32, l+48)
-                const int c1 = __shfl_up(nzc, 16), c2 = __shfl_up(nzc, 32), c3 = __shfl_up(nzc, 48);
-                int pos = (grp >= 1 ? c1 : 0) + (grp >= 2 ? c2 : 0) + (grp >= 3 ? c3 : 0);
-                const int total = __shfl(pos + nzc, (lane & 15) + 48);
-#pragma unroll
-                for (int j = 0; j < kWPer; ++j) {
-                    const int k = ka + grp * kWPer + j;
-                    if (k <= kz && wv[j] != 0.0f) {
-                        wlist[tu_w * kWSlot + pos] = make_int2((k - kc) * kTV, __float_as_int(wv[j]));
-                        ++pos;
-                    }
-                }
-                if (grp == 0) wcnt[tu_w] = ubin ? total : 0;
             }
-            for (int kk0 = wave; kk0 < nk; kk0 += 4 * kStageRows) {
-                if (kk0 != wave) stage_load(kk0);
-                stage_store(kk0);
-            }
-            __syncthreads();
-            if (mrow) {
-#pragma unroll
-                for (int i = 0; i < kTU / 4; ++i) {
-                    const int tu = wave * (kTU / 4) + i;
-                    const int cnt = wcnt[tu];  // wave-uniform
-                    const int2* wl = wlist + tu * kWSlot;
-                    int j = 0;
-                    // offsets (k - kc) * kTV with 0 <= k - kc < nk <= kKMax
-                    for (; j + 1 < cnt; j += 2) {
-                        const int2 e0 = wl[j], e1 = wl[j + 1];
-                        const float p0 = P[e0.x + lane];
-                        const float p1 = P[e1.x + lane];
-                        acc[i] += (double)(__int_as_float(e0.y) * p0);
-                        acc[i] += (double)(__int_as_float(e1.y) * p1);
+            for (int kc = k0t; kc <= k1t; kc += kKMax) {
+                const int nk = min(kKMax, k1t - kc + 1);
+                __syncthreads();  // the previous chunk's reads are done
+                // P rows of this wave's pixels, straight from the volume: the
+                // loads of kStageRows rows are in flight together, the first
+                // group's while the footprint lists below are built.  Every lane
+                // loads: sz[q] = 0 where q >= nzs and the row index is clamped
+                // to the chunk, both valid, and the sums skip those values.
+                float xs[kStageRows][kSlabs];
+                auto stage_load = [&](int kk0) {
+    #pragma unroll
+                    for (int r = 0; r < kStageRows; ++r) {
+                        const int kk = kk0 + kVG * r;
+                        const int k = kc + min(kk, nk - 1);
+                        const int ix = pxd ? k : io, iy = pxd ? io : k;
+                        const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
+    #pragma unroll
+                        for (int q = 0; q < kSlabs; ++q) xs[r][q] = q < nzs_u ? col[sz[q]] : 0.0f;
                     }
-                    if (j < cnt) {
-                        const int2 e0 = wl[j];
-                        acc[i] += (double)(__int_as_float(e0.y) * P[e0.x + lane]);
+                };
+                auto stage_store = [&](int kk0) {
+    #pragma unroll
+                    for (int r = 0; r < kStageRows; ++r) {
+                        const int kk = kk0 + kVG * r;
+                        float sv_ = 0.0f;
+    #pragma unroll
+                        for (int q = 0; q < kSlabs; ++q)
+                            if (q < nzs) sv_ += xs[r][q] * swt[q];
+                        if (kk < nk) P[kk * kTV + lane] = sv_;
+                    }
+                };
+                stage_load(wave);
+                // footprint lists of this wave's bins over the chunk, in pixel
+                // order: lane group grp tests candidates ka + 3 grp .. ka + 3 grp + 2
+                {
+                    const int ka = max(kb0, kc), kz = min(kb1, kc + nk - 1);
+                    float wv[kWPer];
+                    int nzc = 0;
+    #pragma unroll
+                    for (int j = 0; j < kWPer; ++j) {
+                        const int k = ka + grp * kWPer + j;
+                        float w = 0.0f;
+                        if (k <= kz) {
+                            const int ix = xd ? k : io, iy = xd ? io : k;
+                            const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
+                            const float ou = ovl(f.u0, f.u1, ul, ur);
+                            if (ou > 0.0f) w = (ou * inv_du) * f.amp;
+                        }
+                        wv[j] = w;
+                        nzc += (k <= kz && w != 0.0f) ? 1 : 0;
+                    }
+                    // exclusive prefix of the group counts of this bin (lanes l, l+16, l+32, l+48)
+                    const int c1 = __shfl_up(nzc, 16), c2 = __shfl_up(nzc, 32), c3 = __shfl_up(nzc, 48);
+                    int pos = (grp >= 1 ? c1 : 0) + (grp >= 2 ? c2 : 0) + (grp >= 3 ? c3 : 0);
+                    const int total = __shfl(pos + nzc, (lane & 15) + 48);
+    #pragma unroll
+                    for (int j = 0; j < kWPer; ++j) {
+                        const int k = ka + grp * kWPer + j;
+                        if (k <= kz && wv[j] != 0.0f) {
+                            wlist[tu_w * kWSlot + pos] = make_int2((k - kc) * kTV, __float_as_int(wv[j]));
+                            ++pos;
+                        }
+                    }
+                    if (grp == 0) wcnt[tu_w] = ubin ? total : 0;
+                }
+                for (int kk0 = wave; kk0 < nk; kk0 += kVG * kStageRows) {
+                    if (kk0 != wave) stage_load(kk0);
+                    stage_store(kk0);
+                }
+                __syncthreads();
+                if (mrow && active) {
+    #pragma unroll
+                    for (int i = 0; i < kTUB; ++i) {
+                        const int tu = wave * kTUB + i;
+                        const int cnt = wcnt[tu];  // wave-uniform
+                        const int2* wl = wlist + tu * kWSlot;
+                        int j = 0;
+                        // offsets (k - kc) * kTV with 0 <= k - kc < nk <= kKMax
+                        for (; j + 1 < cnt; j += 2) {
+                            const int2 e0 = wl[j], e1 = wl[j + 1];
+                            const float p0 = P[e0.x + lane];
+                            const float p1 = P[e1.x + lane];
+                            acc[i] += (double)(__int_as_float(e0.y) * p0);
+                            acc[i] += (double)(__int_as_float(e1.y) * p1);
+                        }
+                        if (j < cnt) {
+                            const int2 e0 = wl[j];
+                            acc[i] += (double)(__int_as_float(e0.y) * P[e0.x + lane]);
+                        }
                     }
                 }
             }
         }
     }
-    if (!mrow) return;
+    if (!mrow || !vwave) return;
     const int64_t base = (((int64_t)b * G.na + ia) * G.nu) * G.nv + m_lo + lane;
 #pragma unroll
-    for (int i = 0; i < kTU / 4; ++i) {
-        const int tu = wave * (kTU / 4) + i;
-        if (tu >= u_n) continue;
-        const int64_t o = base + (int64_t)(u_lo + tu) * G.nv;
+    for (int i = 0; i < kTUB; ++i) {
+        if (i >= u_n) continue;
+        const int64_t o = base + (int64_t)(u_lo + i) * G.nv;
         float res = (float)acc[i] * G.out_scale;
         if (accumulate) res += sino[o];
         sino[o] = res;
@@ -826,10 +864,11 @@ int vir_sf_fp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
         case SF_CONE3: hipLaunchKernelGGL(k_sf_fp<SF_CONE3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc); break;
         default:
             if (sf_par3_tiled_ok(G) && !(flags & VIR_PER_OUTPUT)) {
-                const int nut = (G.nu + kTU - 1) / kTU, nmt = (G.nv + kTV - 1) / kTV;
-                const int64_t nblk = (int64_t)G.nb * G.na * nut * nmt;
-                hipLaunchKernelGGL(k_sf_fp_par3_tile, dim3((unsigned)nblk), dim3(256), 0, st, G, ang, cosv, sinv,
-                                   img, sino, acc, nut, nmt);
+                const int nut = (G.nu + kTUB - 1) / kTUB, nmt = (G.nv + kTV - 1) / kTV;
+                const int nvg = (G.na + kVG - 1) / kVG;
+                const int64_t nblk = (int64_t)G.nb * nvg * nut * nmt;
+                hipLaunchKernelGGL(k_sf_fp_par3_tile, dim3((unsigned)nblk), dim3(64 * kVG), 0, st, G, ang, cosv, sinv,
+                                   img, sino, acc, nut, nmt, nvg);
             } else {
                 hipLaunchKernelGGL(k_sf_fp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc);
             }
