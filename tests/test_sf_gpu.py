@@ -112,7 +112,8 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     one-thread-per-bin gather (VIR_PER_OUTPUT): several bin tiles and a
     partial one, two row tiles, lines whose pixel band exceeds one staged
     chunk (du = 2 d), a 45-degree view, groups of four views with both
-    driving axes in one group and a last group of one view, a batch of two.
+    driving axes in one group and a last group of one view, a batch of two;
+    and rows as tall as a voxel, offset so that a row tile meets 65 slabs.
     Likewise the column backprojector
     against the one-thread-per-voxel kernel (two slab tiles, 76 views: two
     view groups)."""
@@ -130,6 +131,18 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     _lib.check(_lib.lib().vir_sf_fp(P.g, _lib.ptr(P.ang), _lib.ptr(P.cos), _lib.ptr(P.sin), _lib.ptr(x),
                                     _lib.ptr(gather), P.fp_scale, _lib.VIR_PER_OUTPUT, stream()), "fp")
     a, b = tiled.cpu().numpy(), gather.cpu().numpy()
+    assert np.abs(b).max() > 0
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    # rows as tall as a voxel and offset by 0.3 of one: 64 rows meet 65 slabs
+    shape3, nu3, nv3 = (40, 36, 80), 30, 70
+    ang3 = np.linspace(0, np.pi, 9, endpoint=False)
+    P3 = SFProjector(shape3, ang3, nu3, "par3d", nv=nv3, du=1.0, dv=1.0, su=0.1, sv=0.3, d_pix=1.0)
+    x3 = torch.from_numpy(rng.uniform(0, 1, shape3).astype(np.float32)).cuda()
+    tiled3 = P3.fp(x3)
+    gather3 = torch.empty_like(tiled3)
+    _lib.check(_lib.lib().vir_sf_fp(P3.g, _lib.ptr(P3.ang), _lib.ptr(P3.cos), _lib.ptr(P3.sin), _lib.ptr(x3),
+                                    _lib.ptr(gather3), P3.fp_scale, _lib.VIR_PER_OUTPUT, stream()), "fp")
+    a, b = tiled3.cpu().numpy(), gather3.cpu().numpy()
     assert np.abs(b).max() > 0
     assert np.array_equal(a.view(np.int32), b.view(np.int32))
     # backprojection: column kernel vs per-voxel kernel

@@ -309,14 +309,16 @@ constexpr int kKMax = 128;       // pixels of one line handled at once
 constexpr int kWMax = 12;        // pixels whose footprint meets one bin (checked on the host)
 constexpr int kWSlot = kWMax;    // footprint list stride per bin (LDS: 4 workgroups per CU)
 constexpr int kWPer = 3;         // candidate pixels per lane: 4 lanes x 3 cover kWMax
-constexpr int kStageRows = 3;    // pixel rows whose volume loads are in flight together
+constexpr int kZW = 66;          // slabs a tile's 64 rows can meet (rows no taller than a voxel: 65)
 
 __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
                                                         const float* __restrict__ cs,
                                                         const float* __restrict__ sn,
                                                         const float* __restrict__ img, float* __restrict__ sino,
                                                         int accumulate, int nut, int nmt, int nvg) {
-    __shared__ float P[kKMax * kTV];
+    // row kk of the chunk: the voxels vox[k][zb + z] of pixel k = kc + kk,
+    // then, in place, its slab sums P[k][m] for the tile's rows m
+    __shared__ float R[kKMax * kZW];
     // footprint list per bin: (LDS offset (k - kc) * kTV of the j-th pixel,
     // its weight overlap/du * amplitude as bits), read as one 64-bit word
     __shared__ int2 wlist[kTU * kWSlot];
@@ -376,6 +378,25 @@ __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const flo
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) nzs_max = max(nzs_max, __shfl_xor(nzs_max, m));
     const int nzs_u = __builtin_amdgcn_readfirstlane(nzs_max);
+    // the slabs of all the tile's rows: zb .. zb + span - 1 (the same in every wave)
+    int zlo = 0x7fffffff, zhi = -1;
+#pragma unroll
+    for (int q = 0; q < kSlabs; ++q)
+        if (q < nzs) {
+            zlo = min(zlo, sz[q]);
+            zhi = max(zhi, sz[q]);
+        }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        zlo = min(zlo, __shfl_xor(zlo, m));
+        zhi = max(zhi, __shfl_xor(zhi, m));
+    }
+    const int zb = __builtin_amdgcn_readfirstlane(zhi >= 0 ? zlo : 0);
+    const int span = min(__builtin_amdgcn_readfirstlane(zhi >= 0 ? zhi - zlo + 1 : 0), kZW);
+    // this lane's slabs as offsets into an LDS row
+    int zo[kSlabs];
+#pragma unroll
+    for (int q = 0; q < kSlabs; ++q) zo[q] = min(max(sz[q] - zb, 0), kZW - 1);
 
     // footprint-list lanes: bin u_lo + (lane & 15) of this wave's view (list
     // slot tu_w), candidate group grp
@@ -426,7 +447,7 @@ __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const flo
             const float oc = (sf_bnd(no, G.d, io) + sf_bnd(no, G.d, io + 1)) * 0.5f;
             // the union of the pass's views' pixel bands on this line
             int k0t = nd, k1t = -1;
-    #pragma unroll
+#pragma unroll
             for (int w = 0; w < kVG; ++w) {
                 if (w >= nva || gxd[w] != pxd) continue;
                 const float x0 = __builtin_fmaf(-oc, glin[w], gat0[w]);
@@ -453,42 +474,30 @@ __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const flo
             for (int kc = k0t; kc <= k1t; kc += kKMax) {
                 const int nk = min(kKMax, k1t - kc + 1);
                 __syncthreads();  // the previous chunk's reads are done
-                // P rows of this wave's pixels, straight from the volume: the
-                // loads of kStageRows rows are in flight together, the first
-                // group's while the footprint lists below are built.  Every lane
-                // loads: sz[q] = 0 where q >= nzs and the row index is clamped
-                // to the chunk, both valid, and the sums skip those values.
-                float xs[kStageRows][kSlabs];
-                auto stage_load = [&](int kk0) {
-    #pragma unroll
-                    for (int r = 0; r < kStageRows; ++r) {
-                        const int kk = kk0 + kVG * r;
-                        const int k = kc + min(kk, nk - 1);
-                        const int ix = pxd ? k : io, iy = pxd ? io : k;
-                        const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz;
-    #pragma unroll
-                        for (int q = 0; q < kSlabs; ++q) xs[r][q] = q < nzs_u ? col[sz[q]] : 0.0f;
-                    }
-                };
-                auto stage_store = [&](int kk0) {
-    #pragma unroll
-                    for (int r = 0; r < kStageRows; ++r) {
-                        const int kk = kk0 + kVG * r;
-                        float sv_ = 0.0f;
-    #pragma unroll
-                        for (int q = 0; q < kSlabs; ++q)
-                            if (q < nzs) sv_ += xs[r][q] * swt[q];
-                        if (kk < nk) P[kk * kTV + lane] = sv_;
-                    }
-                };
-                stage_load(wave);
+                // the chunk's voxel rows, straight from the volume into LDS
+                // (consecutive lanes, consecutive z; no registers held): all of
+                // a wave's rows are in flight together, while the footprint
+                // lists below are built
+                for (int kk = wave; kk < nk; kk += kVG) {
+                    const int k = kc + kk;
+                    const int ix = pxd ? k : io, iy = pxd ? io : k;
+                    const float* col = vol + ((int64_t)ix * G.ny + iy) * G.nz + zb;
+                    float* row = R + kk * kZW;
+                    if (lane < span)
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(col + lane),
+                                                         (__attribute__((address_space(3))) void*)row, 4, 0, 0);
+                    if (lane < span - 64)
+                        __builtin_amdgcn_global_load_lds(
+                            (const __attribute__((address_space(1))) void*)(col + 64 + lane),
+                            (__attribute__((address_space(3))) void*)(row + 64), 4, 0, 0);
+                }
                 // footprint lists of this wave's bins over the chunk, in pixel
                 // order: lane group grp tests candidates ka + 3 grp .. ka + 3 grp + 2
                 {
                     const int ka = max(kb0, kc), kz = min(kb1, kc + nk - 1);
                     float wv[kWPer];
                     int nzc = 0;
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < kWPer; ++j) {
                         const int k = ka + grp * kWPer + j;
                         float w = 0.0f;
@@ -505,39 +514,48 @@ __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const flo
                     const int c1 = __shfl_up(nzc, 16), c2 = __shfl_up(nzc, 32), c3 = __shfl_up(nzc, 48);
                     int pos = (grp >= 1 ? c1 : 0) + (grp >= 2 ? c2 : 0) + (grp >= 3 ? c3 : 0);
                     const int total = __shfl(pos + nzc, (lane & 15) + 48);
-    #pragma unroll
+#pragma unroll
                     for (int j = 0; j < kWPer; ++j) {
                         const int k = ka + grp * kWPer + j;
                         if (k <= kz && wv[j] != 0.0f) {
-                            wlist[tu_w * kWSlot + pos] = make_int2((k - kc) * kTV, __float_as_int(wv[j]));
+                            wlist[tu_w * kWSlot + pos] = make_int2((k - kc) * kZW, __float_as_int(wv[j]));
                             ++pos;
                         }
                     }
                     if (grp == 0) wcnt[tu_w] = ubin ? total : 0;
                 }
-                for (int kk0 = wave; kk0 < nk; kk0 += kVG * kStageRows) {
-                    if (kk0 != wave) stage_load(kk0);
-                    stage_store(kk0);
+                __syncthreads();  // the voxel rows have landed
+                // slab sums of this wave's rows, in place: each row is read
+                // and rewritten by one wave, reads before writes
+                for (int kk = wave; kk < nk; kk += kVG) {
+                    float x[kSlabs];
+#pragma unroll
+                    for (int q = 0; q < kSlabs; ++q) x[q] = q < nzs_u ? R[kk * kZW + zo[q]] : 0.0f;
+                    float sv_ = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < kSlabs; ++q)
+                        if (q < nzs) sv_ += x[q] * swt[q];
+                    R[kk * kZW + lane] = sv_;
                 }
                 __syncthreads();
                 if (mrow && active) {
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < kTUB; ++i) {
                         const int tu = wave * kTUB + i;
                         const int cnt = wcnt[tu];  // wave-uniform
                         const int2* wl = wlist + tu * kWSlot;
                         int j = 0;
-                        // offsets (k - kc) * kTV with 0 <= k - kc < nk <= kKMax
+                        // offsets (k - kc) * kZW with 0 <= k - kc < nk <= kKMax
                         for (; j + 1 < cnt; j += 2) {
                             const int2 e0 = wl[j], e1 = wl[j + 1];
-                            const float p0 = P[e0.x + lane];
-                            const float p1 = P[e1.x + lane];
+                            const float p0 = R[e0.x + lane];
+                            const float p1 = R[e1.x + lane];
                             acc[i] += (double)(__int_as_float(e0.y) * p0);
                             acc[i] += (double)(__int_as_float(e1.y) * p1);
                         }
                         if (j < cnt) {
                             const int2 e0 = wl[j];
-                            acc[i] += (double)(__int_as_float(e0.y) * P[e0.x + lane]);
+                            acc[i] += (double)(__int_as_float(e0.y) * R[e0.x + lane]);
                         }
                     }
                 }
