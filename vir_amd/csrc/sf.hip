@@ -307,10 +307,11 @@ constexpr int kTU = kTUB * kVG;  // footprint lists per workgroup
 constexpr int kTV = 64;          // detector rows per tile (one per lane)
 constexpr int kKMax = 128;       // pixels of one line handled at once
 constexpr int kWMax = 12;        // pixels whose footprint meets one bin (checked on the host)
-constexpr int kWSlot = kWMax;    // footprint list stride per bin (LDS: 4 workgroups per CU)
-constexpr int kWPer = 3;         // candidate pixels per lane: 4 lanes x 3 cover kWMax
+// WPER candidate pixels per lane, 4 lanes per bin: 4 WPER <= kWMax pixels per
+// bin and line (the host picks the smallest WPER its bound allows)
 constexpr int kZW = 66;          // slabs a tile's 64 rows can meet (rows no taller than a voxel: 65)
 
+template <int WPER>
 __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
                                                         const float* __restrict__ cs,
                                                         const float* __restrict__ sn,
@@ -321,6 +322,7 @@ __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const flo
     __shared__ float R[kKMax * kZW];
     // footprint list per bin: (LDS offset (k - kc) * kTV of the j-th pixel,
     // its weight overlap/du * amplitude as bits), read as one 64-bit word
+    constexpr int kWSlot = 4 * WPER;  // footprint list stride per bin
     __shared__ int2 wlist[kTU * kWSlot];
     __shared__ int wcnt[kTU];
     const int lane = threadIdx.x & 63;
@@ -492,14 +494,14 @@ __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const flo
                             (__attribute__((address_space(3))) void*)(row + 64), 4, 0, 0);
                 }
                 // footprint lists of this wave's bins over the chunk, in pixel
-                // order: lane group grp tests candidates ka + 3 grp .. ka + 3 grp + 2
+                // order: lane group grp tests candidates ka + WPER grp .. ka + WPER grp + WPER - 1
                 {
                     const int ka = max(kb0, kc), kz = min(kb1, kc + nk - 1);
-                    float wv[kWPer];
+                    float wv[WPER];
                     int nzc = 0;
 #pragma unroll
-                    for (int j = 0; j < kWPer; ++j) {
-                        const int k = ka + grp * kWPer + j;
+                    for (int j = 0; j < WPER; ++j) {
+                        const int k = ka + grp * WPER + j;
                         float w = 0.0f;
                         if (k <= kz) {
                             const int ix = xd ? k : io, iy = xd ? io : k;
@@ -515,8 +517,8 @@ __global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const flo
                     int pos = (grp >= 1 ? c1 : 0) + (grp >= 2 ? c2 : 0) + (grp >= 3 ? c3 : 0);
                     const int total = __shfl(pos + nzc, (lane & 15) + 48);
 #pragma unroll
-                    for (int j = 0; j < kWPer; ++j) {
-                        const int k = ka + grp * kWPer + j;
+                    for (int j = 0; j < WPER; ++j) {
+                        const int k = ka + grp * WPER + j;
                         if (k <= kz && wv[j] != 0.0f) {
                             wlist[tu_w * kWSlot + pos] = make_int2((k - kc) * kZW, __float_as_int(wv[j]));
                             ++pos;
@@ -728,13 +730,20 @@ static bool sf_par3_bp_col_ok(const SFGeo& G) {
     return 1.4143 * (double)G.d / (double)G.du + 4.0 <= (double)kJMax;
 }
 
+// Bound on the candidate pixels of one bin along a line: a bin's pixel range
+// is floor(b1) - floor(b0) + 4 long, b1 - b0 = (du + 2 wmax) / (|pd| d) with
+// |pd| >= 1/sqrt 2, so it is at most floor(X) + 5 for X that ratio's bound;
+// per_bin = X + 5 keeps a margin for rounding.
+static double sf_par3_per_bin(const SFGeo& G) {
+    return ((double)G.du + 2.0 * (double)G.wmax) / (0.70710678 * (double)G.d) + 5.0;
+}
+
 // True when the tiled parallel 3-D forward kernel applies: detector rows no
 // taller than a voxel (each row meets at most kSlabs slabs) and at most
 // kWMax - 1 pixels per bin along a line.
 static bool sf_par3_tiled_ok(const SFGeo& G) {
     if (!(G.dv <= G.d)) return false;
-    const double per_bin = ((double)G.du + 2.0 * (double)G.wmax) / (0.70710678 * (double)G.d) + 5.0;
-    return per_bin <= (double)(kWMax - 1);
+    return sf_par3_per_bin(G) <= (double)(kWMax - 1);
 }
 
 template <int MODE>
@@ -885,8 +894,14 @@ int vir_sf_fp(const vir_sf_geom* geom, const float* ang, const float* cosv, cons
                 const int nut = (G.nu + kTUB - 1) / kTUB, nmt = (G.nv + kTV - 1) / kTV;
                 const int nvg = (G.na + kVG - 1) / kVG;
                 const int64_t nblk = (int64_t)G.nb * nvg * nut * nmt;
-                hipLaunchKernelGGL(k_sf_fp_par3_tile, dim3((unsigned)nblk), dim3(64 * kVG), 0, st, G, ang, cosv, sinv,
-                                   img, sino, acc, nut, nmt, nvg);
+                // two candidates per lane when a bin's range is at most 8 pixels
+                // (X <= 3.9 < 4), else three (X <= 6: at most 11 of 12)
+                if (sf_par3_per_bin(G) <= 8.9)
+                    hipLaunchKernelGGL(k_sf_fp_par3_tile<2>, dim3((unsigned)nblk), dim3(64 * kVG), 0, st, G, ang, cosv,
+                                       sinv, img, sino, acc, nut, nmt, nvg);
+                else
+                    hipLaunchKernelGGL(k_sf_fp_par3_tile<3>, dim3((unsigned)nblk), dim3(64 * kVG), 0, st, G, ang, cosv,
+                                       sinv, img, sino, acc, nut, nmt, nvg);
             } else {
                 hipLaunchKernelGGL(k_sf_fp<SF_PAR3>, grd, dim3(256), 0, st, G, ang, cosv, sinv, img, sino, acc);
             }
