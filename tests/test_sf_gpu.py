@@ -116,7 +116,8 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     and rows as tall as a voxel, offset so that a row tile meets 65 slabs.
     Likewise the column backprojector
     against the one-thread-per-voxel kernel (two slab tiles, 76 views: two
-    view groups)."""
+    view groups), with rows shorter than a slab and with rows aligned with
+    the slabs."""
     import torch
     from vir_amd import _lib
     from vir_amd.device import stream
@@ -155,6 +156,17 @@ def test_par3d_tiled_forward_equals_per_bin_gather():
     _lib.check(_lib.lib().vir_sf_bp(P2.g, _lib.ptr(P2.ang), _lib.ptr(P2.cos), _lib.ptr(P2.sin), _lib.ptr(y),
                                     _lib.ptr(vox), P2.fp_scale, _lib.VIR_PER_OUTPUT, stream()), "bp")
     a, b = col.cpu().numpy(), vox.cpu().numpy()
+    assert np.abs(b).max() > 0
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    # rows aligned with the slabs (each slab meets one row): the one-row form
+    shape4, nu4, nv4 = (24, 20, 70), 40, 70
+    P4 = SFProjector(shape4, ang2, nu4, "par3d", nv=nv4, du=0.9, dv=1.0, su=0.2, d_pix=1.0, batch=2)
+    y4 = torch.from_numpy(rng.normal(size=P4.sino_shape).astype(np.float32)).cuda()
+    col4 = P4.bp(y4)
+    vox4 = torch.empty_like(col4)
+    _lib.check(_lib.lib().vir_sf_bp(P4.g, _lib.ptr(P4.ang), _lib.ptr(P4.cos), _lib.ptr(P4.sin), _lib.ptr(y4),
+                                    _lib.ptr(vox4), P4.fp_scale, _lib.VIR_PER_OUTPUT, stream()), "bp")
+    a, b = col4.cpu().numpy(), vox4.cpu().numpy()
     assert np.abs(b).max() > 0
     assert np.array_equal(a.view(np.int32), b.view(np.int32))
 

@@ -23,6 +23,7 @@
 // amplitude takes its lower y boundary from just after the previous nonzero
 // pixel of the column (the first boundary if none); footprints are unchanged.
 #include <cmath>
+#include <type_traits>
 
 #include "vir_common.h"
 
@@ -637,84 +638,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     int nzr_max = nzr;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) nzr_max = max(nzr_max, __shfl_xor(nzr_max, m));
-    const bool pre = __builtin_amdgcn_readfirstlane(nzr_max) <= 2;
+    const int nzr_u = __builtin_amdgcn_readfirstlane(nzr_max);
+    const bool pre = nzr_u <= 2;
 
     const float u0 = sf_det(G.nu, G.du, G.su, 0);
     const float inv_du = 1.0f / G.du;
     const float* sb = sino + (int64_t)b * G.na * G.nu * G.nv;
     double acc = 0.0;
-    for (int va = 0; va < G.na; va += 64) {
-        // lane = view va + lane: bins of this pixel's footprint with positive overlap
-        int cnt = 0, jl[kJMax];
-        float wl[kJMax];
-#pragma unroll
-        for (int s = 0; s < kJMax; ++s) {
-            jl[s] = 0;
-            wl[s] = 0.0f;
-        }
-        const int ial = va + lane;
-        if (ial < G.na) {
-            const SFView V = sf_view(G, cs[ial], sn[ial], ang[ial]);
-            const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
-            if (f.u1 > u0 - G.du && (f.u0 - u0) * inv_du < (float)G.nu + 1.0f) {
-                const int j0 = (int)fmaxf(floorf((f.u0 - u0) * inv_du) - 1.0f, 0.0f);
-                const int j1 = (int)fminf(floorf((f.u1 - u0) * inv_du) + 1.0f, (float)(G.nu - 1));
-#pragma unroll
-                for (int t = 0; t < kJMax; ++t) {
-                    const int j = j0 + t;
-                    if (j > j1) break;
-                    const float ou = ovl(f.u0, f.u1, sf_det(G.nu, G.du, G.su, j), sf_det(G.nu, G.du, G.su, j + 1));
-                    if (!(ou > 0.0f)) continue;
-                    const float wu = (ou * inv_du) * f.amp;
-#pragma unroll
-                    for (int s = 0; s < kJMax; ++s) {
-                        if (s == cnt) {
-                            jl[s] = j;
-                            wl[s] = wu;
-                        }
-                    }
-                    ++cnt;
-                }
-            }
-        }
-        const int nvw = min(64, G.na - va);
-        for (int v = 0; v < nvw; ++v) {
-            const int c = __builtin_amdgcn_readlane(cnt, v);
-            const float* plane = sb + (int64_t)(va + v) * G.nu * G.nv;
-            // the rows of the first kJPre bins, two per slab, are loaded
-            // together before any is used, by every lane and for every one
-            // of these bins: bins past the footprint's count read bin jl = 0,
-            // rows past the slab's count row rq = 0, both valid, and the sums
-            // below skip them
-            float xv[kJPre][2];
-            if (pre) {
-#pragma unroll
-                for (int s = 0; s < kJPre; ++s) {
-                    const float* row = plane + (int64_t)__builtin_amdgcn_readlane(jl[s], v) * G.nv;
-                    xv[s][0] = row[rq[0]];
-                    xv[s][1] = row[rq[1]];
-                }
-            }
+    // the view loop, for NR preloaded rows per bin: one where no slab of the
+    // wave meets a second row (rows aligned with the slabs), else two
+    auto views = [&](auto nr_tag) {
+        constexpr int NR = decltype(nr_tag)::value;
+        for (int va = 0; va < G.na; va += 64) {
+            // lane = view va + lane: bins of this pixel's footprint with positive overlap
+            int cnt = 0, jl[kJMax];
+            float wl[kJMax];
 #pragma unroll
             for (int s = 0; s < kJMax; ++s) {
-                if (s >= c) break;
-                const float wu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wl[s]), v));
-                float sv_ = 0.0f;
-                if (s < kJPre && pre) {
-                    const int sp = s < kJPre ? s : 0;
-                    sv_ = 0 < nzr ? sv_ + xv[sp][0] * rw[0] : sv_;
-                    sv_ = 1 < nzr ? sv_ + xv[sp][1] * rw[1] : sv_;
-                } else {
-                    const int j = __builtin_amdgcn_readlane(jl[s], v);
-                    const float* row = plane + (int64_t)j * G.nv;
+                jl[s] = 0;
+                wl[s] = 0.0f;
+            }
+            const int ial = va + lane;
+            if (ial < G.na) {
+                const SFView V = sf_view(G, cs[ial], sn[ial], ang[ial]);
+                const SFFoot f = sf_foot<SF_PAR3>(G, V, ix, iy);
+                if (f.u1 > u0 - G.du && (f.u0 - u0) * inv_du < (float)G.nu + 1.0f) {
+                    const int j0 = (int)fmaxf(floorf((f.u0 - u0) * inv_du) - 1.0f, 0.0f);
+                    const int j1 = (int)fminf(floorf((f.u1 - u0) * inv_du) + 1.0f, (float)(G.nu - 1));
 #pragma unroll
-                    for (int q = 0; q < kSlabs; ++q)
-                        if (q < nzr) sv_ += row[rq[q]] * rw[q];
+                    for (int t = 0; t < kJMax; ++t) {
+                        const int j = j0 + t;
+                        if (j > j1) break;
+                        const float ou = ovl(f.u0, f.u1, sf_det(G.nu, G.du, G.su, j), sf_det(G.nu, G.du, G.su, j + 1));
+                        if (!(ou > 0.0f)) continue;
+                        const float wu = (ou * inv_du) * f.amp;
+#pragma unroll
+                        for (int s = 0; s < kJMax; ++s) {
+                            if (s == cnt) {
+                                jl[s] = j;
+                                wl[s] = wu;
+                            }
+                        }
+                        ++cnt;
+                    }
                 }
-                acc += (double)(wu * sv_);
+            }
+            const int nvw = min(64, G.na - va);
+            for (int v = 0; v < nvw; ++v) {
+                const int c = __builtin_amdgcn_readlane(cnt, v);
+                const float* plane = sb + (int64_t)(va + v) * G.nu * G.nv;
+                // the rows of the first kJPre bins, two per slab, are loaded
+                // together before any is used, by every lane and for every one
+                // of these bins: bins past the footprint's count read bin jl = 0,
+                // rows past the slab's count row rq = 0, both valid, and the sums
+                // below skip them
+                float xv[kJPre][2];
+                if (pre) {
+#pragma unroll
+                    for (int s = 0; s < kJPre; ++s) {
+                        const float* row = plane + (int64_t)__builtin_amdgcn_readlane(jl[s], v) * G.nv;
+                        xv[s][0] = row[rq[0]];
+                        xv[s][1] = NR > 1 ? row[rq[1]] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < kJMax; ++s) {
+                    if (s >= c) break;
+                    const float wu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wl[s]), v));
+                    float sv_ = 0.0f;
+                    if (s < kJPre && pre) {
+                        const int sp = s < kJPre ? s : 0;
+                        sv_ = 0 < nzr ? sv_ + xv[sp][0] * rw[0] : sv_;
+                        if (NR > 1) sv_ = 1 < nzr ? sv_ + xv[sp][1] * rw[1] : sv_;
+                    } else {
+                        const int j = __builtin_amdgcn_readlane(jl[s], v);
+                        const float* row = plane + (int64_t)j * G.nv;
+#pragma unroll
+                        for (int q = 0; q < kSlabs; ++q)
+                            if (q < nzr) sv_ += row[rq[q]] * rw[q];
+                    }
+                    acc += (double)(wu * sv_);
+                }
             }
         }
-    }
+    };
+    if (nzr_u <= 1)
+        views(std::integral_constant<int, 1>{});
+    else
+        views(std::integral_constant<int, 2>{});
     if (!zin) return;
     const int64_t o = (((int64_t)b * G.nx + ix) * G.ny + iy) * G.nz + iz;
     float res = (float)acc * G.out_scale;
