@@ -306,14 +306,14 @@ constexpr int kTUB = 16;         // detector bins per tile and view
 constexpr int kVG = 4;           // views per workgroup (one per wave)
 constexpr int kTU = kTUB * kVG;  // footprint lists per workgroup
 constexpr int kTV = 64;          // detector rows per tile (one per lane)
-constexpr int kKMax = 128;       // pixels of one line handled at once
+constexpr int kKMax = 80;        // pixels of one line handled at once (LDS: 5-6 workgroups per CU)
 constexpr int kWMax = 12;        // pixels whose footprint meets one bin (checked on the host)
 // WPER candidate pixels per lane, 4 lanes per bin: 4 WPER <= kWMax pixels per
 // bin and line (the host picks the smallest WPER its bound allows)
 constexpr int kZW = 66;          // slabs a tile's 64 rows can meet (rows no taller than a voxel: 65)
 
 template <int WPER>
-__global__ __launch_bounds__(64 * kVG) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
+__global__ __launch_bounds__(64 * kVG) __attribute__((amdgpu_waves_per_eu(6))) void k_sf_fp_par3_tile(SFGeo G, const float* __restrict__ ang,
                                                         const float* __restrict__ cs,
                                                         const float* __restrict__ sn,
                                                         const float* __restrict__ img, float* __restrict__ sino,
